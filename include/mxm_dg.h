@@ -1,0 +1,190 @@
+/*
+ * mxm_dg.h -- C ABI of libmxm_hip.so, the MI355X (gfx950) implementation of
+ * mojoxm's DG right-hand side and SSPRK3 time step.
+ *
+ * Conventions follow the reference's only native shim, src/mpi_shim.c:
+ *   - every export is `int mxm_<area>_<verb>(...)`, plain C, no C++ types;
+ *   - arguments are int / long long / double, pointers to plain structs
+ *     declared here, void* handles and void** out-handles
+ *     (src/mpi_shim.c:82-98, src/mpi.mojo:93-97);
+ *   - 0 means success, a negative code an error; mxm_last_error() returns
+ *     the message of the last failure on the calling thread
+ *     (src/mpi.mojo:35-37 raises on rc != 0 the same way);
+ *   - one host thread drives a handle (src/mpi_shim.c:41-45, FUNNELED);
+ *   - the caller owns host buffers; the library owns all device memory and
+ *     its stream.  Uploads/downloads are synchronous; stage and step calls
+ *     are asynchronous until a download, mxm_dg_sync or a bench call.
+ *
+ * State layout (host and device): element-major, q[(e*NP + i)*NC + c]
+ * (src/solver.mojo:17-21), FP64 on the host side of the ABI.
+ */
+#ifndef MXM_DG_H
+#define MXM_DG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define MXM_OK 0
+#define MXM_ERR_ARG -1    /* bad argument / unsupported combination        */
+#define MXM_ERR_HIP -2    /* a HIP runtime call failed                     */
+#define MXM_ERR_NOGPU -3  /* no device / mxm_hip_init not called           */
+#define MXM_ERR_MESH -4   /* mesh construction or validation failed        */
+#define MXM_ERR_ALLOC -5  /* host or device allocation failed              */
+
+/* ---- boundary kinds, src/boundary.mojo:24-27 --------------------------- */
+#define MXM_BC_INTERIOR 0 /* periodic / two-sided face */
+#define MXM_BC_WALL 1
+#define MXM_BC_OUTFLOW 2
+#define MXM_BC_INFLOW 3
+
+/* ---- physics ids and numerical flux kinds ----------------------------- */
+#define MXM_PHYS_ADVECTION 0   /* trait Advection, src/advection.mojo, NC=1      */
+#define MXM_PHYS_EULER 1       /* trait Euler, src/euler.mojo, NC=5              */
+#define MXM_PHYS_EULER2D 2     /* src/local_mesh_2d_gpu_euler.mojo, NC=4 (2D)    */
+#define MXM_PHYS_SW2D 3        /* src/local_mesh_2d_gpu_sw.mojo, NC=3 (2D)       */
+#define MXM_PHYS_MHDGLM2D 4    /* src/local_mesh_2d_gpu_mhd_glm.mojo, NC=7 (2D)  */
+#define MXM_PHYS_MAXWELL2D 5   /* src/local_mesh_2d_gpu_maxwell.mojo, NC=6 (2D)  */
+#define MXM_PHYS_ADVECTION2D 6 /* src/local_mesh_2d_gpu_advection.mojo, NC=1 (2D)*/
+
+/* Euler flux kinds, src/euler.mojo:33-36 */
+#define MXM_FLUX_RUSANOV 0
+#define MXM_FLUX_ROE 1
+#define MXM_FLUX_HLLE 2
+#define MXM_FLUX_HLLEC 3
+
+/* Physics parameters travel as double prm[MXM_PRM_MAX]; index layout:
+ *  ADVECTION   [0..2] v, [3] inflow_q                   (src/advection.mojo:24-40)
+ *  EULER       [0] gamma [1] min_density [2] min_pressure [3] flux kind
+ *              [4] entropy_fix [5..7] gravity [8..12] inflow (rho, rhou,
+ *              rhov, rhow, E)                            (src/euler.mojo:538-575)
+ *  EULER2D     [0] gamma [1] min_density [2] min_pressure [3] 0=Rusanov
+ *              1=HLLC [4..5] gravity [6..9] inflow (rho, rhou, rhov, E)
+ *  SW2D        [0] g [1] min_h [2] 0=Rusanov 1=HLL [3..5] inflow (h, hu, hv)
+ *  MHDGLM2D    [0] gamma [1] min_density [2] min_pressure [3] c_h
+ *              [4] alpha_d [5..11] inflow (rho, rhou, rhov, Bx, By, E, psi)
+ *  MAXWELL2D   [0] c [1..3] J [4..6] M [7..12] inflow (Ex, Ey, Ez, Bx, By, Bz)
+ *  ADVECTION2D [0..1] v [2] inflow_q
+ */
+#define MXM_PRM_MAX 16
+
+/* Arithmetic of the stage kernels. */
+#define MXM_PREC_FP64 0 /* plain FP64 state, geometry and operators */
+
+/* ---- mesh descriptors -------------------------------------------------- */
+/* bc order [-x, +x, -y, +y, -z, +z], src/boundary.mojo:40-56 */
+typedef struct {
+    int nx, ny, nz;
+    double lx, ly, lz;
+    int bc[6];
+} mxm_mesh3d_desc;
+
+/* bc order [-x, +x, -y, +y], src/boundary.mojo:64-76 */
+typedef struct {
+    int nx, ny;
+    double lx, ly;
+    int bc[4];
+} mxm_mesh2d_desc;
+
+/* Mesh tables in the reference's layout (src/local_mesh.mojo:1195-1236,
+ * src/local_mesh_2d.mojo:24-40), for mxm_mesh_get. */
+#define MXM_T_ELEM_NODE_XYZ 0     /* double [K*NP*ndim]     */
+#define MXM_T_ELEM_INVJ 1         /* double [K*ndim*ndim]   */
+#define MXM_T_ELEM_INV_VOL 2      /* double [K] 1/(6V), 1/(2A) */
+#define MXM_T_ELEM_FACES 3        /* int    [K*NF]          */
+#define MXM_T_ELEM_FACE_SIDE 4    /* int    [K*NF]          */
+#define MXM_T_ELEM_CANON_TO_REF 5 /* int    [K*NF*NFP]      */
+#define MXM_T_FACE_ELEM 6         /* int    [Nf*2]          */
+#define MXM_T_FACE_ELEM_NODE 7    /* int    [Nf*2*NFP]      */
+#define MXM_T_FACE_NORMAL 8       /* double [Nf*ndim]       */
+#define MXM_T_FACE_AREA 9         /* double [Nf]            */
+#define MXM_T_FACE_BC_TYPE 10     /* int    [Nf]            */
+
+/* ThroughputReport, src/memory_report.mojo:139-191 */
+typedef struct {
+    int steps;
+    double wall_seconds;
+    long long dof;                  /* K*NP*NC, src/solver.mojo:757-763 */
+    long long state_bytes_per_step; /* 8 touches * K*NP*NC * sizeof(real) */
+    double dof_per_second;          /* dof*steps/wall                   */
+    double dof_stage_updates_per_second; /* 3 * dof_per_second          */
+    double state_bandwidth_gbs;     /* lower-bound state GB/s           */
+} mxm_throughput;
+
+/* MemoryReport, src/memory_report.mojo:36-130 (six categories, bytes). */
+typedef struct {
+    long long rk_stage;
+    long long dg_operators;
+    long long limiter;
+    long long mesh_connectivity;
+    long long halo_device;
+    long long halo_pinned;
+} mxm_memory_report;
+
+/* ---- library ------------------------------------------------------------ */
+/* Select the device for this process; replaces mxm_mpi_init's role of
+ * process setup (src/mpi_shim.c:41-50). */
+int mxm_hip_init(int device);
+const char *mxm_last_error(void);
+/* Constants by getter, as src/mpi_shim.c:148-160 does for MPI handles:
+ * which = 0 -> MXM_PRM_MAX, 1 -> number of device kernels compiled in. */
+int mxm_get_constant(int which, int *value);
+
+/* ---- host-only builders (no GPU needed) ---------------------------------- */
+/* ReferenceElement[P], src/reference.mojo:519-660: D[3*NP*NP],
+ * Lift[4*NP*NFP], w[NP], f2e[4*NFP] (any pointer may be NULL). */
+int mxm_refelem_3d(int P, double *D, double *Lift, double *w, int *f2e);
+/* ReferenceElement2D[P], src/reference_2d.mojo:130-261: D[2*NP*NP],
+ * Lift[3*NP*(P+1)], w[NP], e2e[3*(P+1)]. */
+int mxm_refelem_2d(int P, double *D, double *Lift, double *w, int *e2e);
+/* LocalMesh[P] (src/local_mesh.mojo:1241-1485) / LocalMesh2D[P]
+ * (src/local_mesh_2d.mojo:103-395) built on the host. */
+int mxm_mesh_create_3d(const mxm_mesh3d_desc *desc, int P, void **mesh);
+int mxm_mesh_create_2d(const mxm_mesh2d_desc *desc, int P, void **mesh);
+/* out[0..5] = ndim, K, Nf, NP, NFP, NF */
+int mxm_mesh_info(void *mesh, int *out6);
+int mxm_mesh_get(void *mesh, int table, void *out);
+int mxm_mesh_destroy(void *mesh);
+
+/* ---- solver (replaces Solver[PhysT, P], src/solver.mojo:538-1093, and the
+ *      2D <phys>_rk_stage_2d drivers) ------------------------------------- */
+int mxm_dg_create_3d(const mxm_mesh3d_desc *desc, int P, int phys, const double *prm,
+                     int precision, void **solver);
+int mxm_dg_create_2d(const mxm_mesh2d_desc *desc, int P, int phys, const double *prm,
+                     int precision, void **solver);
+/* out[0..4] = ndim, K, NP, NC, Nf */
+int mxm_dg_info(void *solver, int *out5);
+/* Copy the whole state d_q from/to the host (n = K*NP*NC doubles). */
+int mxm_dg_upload_q(void *solver, const double *q, long long n);
+int mxm_dg_download_q(void *solver, double *q, long long n);
+/* download_owned_component, src/solver.mojo:711-729: out[K*NP]. */
+int mxm_dg_download_component(void *solver, int c, double *out, long long n);
+/* One Shu-Osher stage (0, 1, 2) of the plan in src/ssprk3.mojo:41-78,
+ * followed by the limiter when it is enabled (src/solver.mojo:1027-1036). */
+int mxm_dg_rk_stage(void *solver, int stage, double dt);
+/* nsteps x step_ssprk3, src/solver.mojo:1071-1092 */
+int mxm_dg_step_ssprk3(void *solver, double dt, int nsteps);
+/* enable_cell_limiter, src/solver.mojo:735-751 (eps: Venkatakrishnan). */
+int mxm_dg_enable_limiter(void *solver, int on, double eps);
+int mxm_dg_sync(void *solver);
+/* bench_step_loop, src/solver.mojo:765-802: warmup steps, sync, measure
+ * steps, sync; advances the state by (warmup+measure)*dt. */
+int mxm_dg_bench_step_loop(void *solver, double dt, int warmup, int measure,
+                           mxm_throughput *out);
+/* dof_count / state_bytes_per_step / memory_report,
+ * src/solver.mojo:757-914 (FP64 state: 8 bytes per value). */
+int mxm_dg_dof_count(void *solver, long long *out);
+int mxm_dg_state_bytes_per_step(void *solver, long long *out);
+int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
+/* Average duration (microseconds) of the stage kernel over nsteps SSPRK3
+ * steps, timed with HIP events on the solver's own stream; advances the
+ * state.  No reference counterpart: it is the measurement hook that the
+ * roofline figures are computed from. */
+int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
+int mxm_dg_destroy(void *solver);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

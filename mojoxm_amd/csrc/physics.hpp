@@ -1,0 +1,630 @@
+// physics.hpp -- pointwise physics of the DG right-hand side as
+// __host__ __device__ functors, one per physics kind.
+//
+// Interface (the Physics trait of src/solver.mojo:97-145):
+//   NC                        number of conserved components
+//   flux<ND>(q, F)            F[d*NC + c], world-frame flux, d < ND
+//   numerical_flux(ql,qr,n,f) two-state flux, n points from ql's element
+//   boundary_flux(q,bc,n,f)   flux on a boundary face, n outward
+//   source(q, s)              pointwise source (kSource)
+//   limit(q)                  post-stage positivity fix (kLimit)
+// Trait physics (Advection, Euler) run in 3D and, with nz = 0, on
+// triangles; the 2D module physics (Euler2D, SW, GLM-MHD, Maxwell,
+// Advection2D) follow their reference kernels' inline physics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define MXM_HD __host__ __device__ __forceinline__
+
+namespace mxm {
+
+enum : int { BC_INTERIOR = 0, BC_WALL = 1, BC_OUTFLOW = 2, BC_INFLOW = 3 };
+
+template <class R>
+MXM_HD R rmax(R a, R b) { return a > b ? a : b; }
+template <class R>
+MXM_HD R rmin(R a, R b) { return a < b ? a : b; }
+template <class R>
+MXM_HD R rabs(R a) { return a >= R(0) ? a : -a; }
+
+// ------------------------------------------------------------ Advection --
+// src/advection.mojo:60-119
+template <class R>
+struct Advection {
+  static constexpr int NC = 1;
+  static constexpr bool kSource = false, kLimit = false;
+  R v[3];
+  R inflow;
+  static Advection from(const double *p) { return {{R(p[0]), R(p[1]), R(p[2])}, R(p[3])}; }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    for (int d = 0; d < ND; ++d) F[d] = v[d] * q[0];
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+    R a = rabs(vn);
+    f[0] = R(0.5) * ((vn + a) * ql[0] + (vn - a) * qr[0]);
+  }
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+    R a = rabs(vn);
+    R g = bc == BC_INFLOW ? inflow : R(0);
+    f[0] = R(0.5) * ((vn + a) * q[0] + (vn - a) * g);
+  }
+  MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
+  MXM_HD void limit(R *) const {}
+};
+
+// ---------------------------------------------------------------- Euler --
+// src/euler.mojo:44-792.  FLUX: 0 Rusanov, 1 Roe, 2 HLLE, 3 HLLEC.
+template <class R, int FLUX>
+struct Euler {
+  static constexpr int NC = 5;
+  static constexpr bool kSource = true, kLimit = true;
+  R gamma, rmin_, pmin_;
+  int efix;
+  R g[3];
+  R inflow[5];
+  static Euler from(const double *p) {
+    Euler e;
+    e.gamma = R(p[0]);
+    e.rmin_ = R(p[1]);
+    e.pmin_ = R(p[2]);
+    e.efix = p[4] != 0.0;
+    for (int d = 0; d < 3; ++d) e.g[d] = R(p[5 + d]);
+    for (int c = 0; c < 5; ++c) e.inflow[c] = R(p[8 + c]);
+    return e;
+  }
+  MXM_HD R rho(const R *q) const { return q[0] > rmin_ ? q[0] : rmin_; }
+  MXM_HD R pres(const R *q) const {
+    R r = rho(q);
+    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / r;
+    R p = (gamma - R(1)) * (q[4] - ke);
+    return p > pmin_ ? p : pmin_;
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    R r = rho(q), p = pres(q);
+    R u[3] = {q[1] / r, q[2] / r, q[3] / r};
+    for (int d = 0; d < ND; ++d) {
+      R *Fd = F + d * 5;
+      Fd[0] = q[1 + d];
+      Fd[1] = q[1 + d] * u[0];
+      Fd[2] = q[1 + d] * u[1];
+      Fd[3] = q[1 + d] * u[2];
+      Fd[1 + d] += p;
+      Fd[4] = u[d] * (q[4] + p);
+    }
+  }
+  // x-direction flux of a face-aligned state
+  MXM_HD void flux1(const R *q, R *f) const {
+    R r = rho(q), p = pres(q), u = q[1] / r;
+    f[0] = q[1];
+    f[1] = q[1] * u + p;
+    f[2] = q[2] * u;
+    f[3] = q[3] * u;
+    f[4] = u * (q[4] + p);
+  }
+  struct Frame {
+    R n[3], t[3], b[3];
+  };
+  // tangent / binormal, src/euler.mojo:94-118
+  MXM_HD static Frame frame(const R *n) {
+    Frame F;
+    R a[3] = {n[0], n[1], n[2]};
+    if (rabs(a[0]) >= R(0.9))
+      a[1] += R(1);
+    else
+      a[0] += R(1);
+    R pr = n[0] * a[0] + n[1] * a[1] + n[2] * a[2];
+    for (int d = 0; d < 3; ++d) a[d] -= n[d] * pr;
+    R mag = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    for (int d = 0; d < 3; ++d) {
+      F.n[d] = n[d];
+      F.t[d] = a[d] / mag;
+    }
+    F.b[0] = n[1] * F.t[2] - n[2] * F.t[1];
+    F.b[1] = n[2] * F.t[0] - n[0] * F.t[2];
+    F.b[2] = n[0] * F.t[1] - n[1] * F.t[0];
+    return F;
+  }
+  MXM_HD static void to_local(const Frame &F, const R *q, R *r) {
+    r[0] = q[0];
+    r[4] = q[4];
+    r[1] = F.n[0] * q[1] + F.n[1] * q[2] + F.n[2] * q[3];
+    r[2] = F.t[0] * q[1] + F.t[1] * q[2] + F.t[2] * q[3];
+    r[3] = F.b[0] * q[1] + F.b[1] * q[2] + F.b[2] * q[3];
+  }
+  MXM_HD static void to_world(const Frame &F, const R *r, R *q) {
+    q[0] = r[0];
+    q[4] = r[4];
+    for (int d = 0; d < 3; ++d) q[1 + d] = F.n[d] * r[1] + F.t[d] * r[2] + F.b[d] * r[3];
+  }
+  // Roe averages, src/euler.mojo:161-176 (returns u, v, w, H, a)
+  MXM_HD void roe_avg(const R *ql, const R *qr, R pl, R pr, R *o) const {
+    R sl = sqrt(rho(ql)), sr = sqrt(rho(qr)), s = sl + sr;
+    o[0] = (ql[1] / sl + qr[1] / sr) / s;
+    o[1] = (ql[2] / sl + qr[2] / sr) / s;
+    o[2] = (ql[3] / sl + qr[3] / sr) / s;
+    o[3] = ((ql[4] + pl) / sl + (qr[4] + pr) / sr) / s;
+    R u2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    o[4] = sqrt(rmax((gamma - R(1)) * (o[3] - R(0.5) * u2), R(0)));
+  }
+  // Harten-Hyman entropy fix, src/euler.mojo:207-288.  wave[m*3 + w].
+  MXM_HD void entropy_fixed(const R *ql, const R *qr, const R *s, const R *wave, R *amdq,
+                            R *apdq) const {
+    const R gm = gamma;
+    R rl = rho(ql), pl = pres(ql);
+    R s0 = ql[1] / rl - sqrt(gm * pl / rl);
+    for (int m = 0; m < 5; ++m) amdq[m] = R(0);
+    if (!(s0 > R(0) && s[0] > R(0))) {
+      R a[5];
+      for (int m = 0; m < 5; ++m) a[m] = ql[m] + wave[m * 3];
+      a[0] = rmax(a[0], rmin_);
+      R ke = R(0.5) * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / a[0];
+      R p1 = rmax((gm - R(1)) * (a[4] - ke), pmin_);
+      R s1 = a[1] / a[0] - sqrt(gm * p1 / a[0]);
+      R sf = (s0 < R(0) && s1 > R(0)) ? s0 * (s1 - s[0]) / (s1 - s0) : (s[0] < R(0) ? s[0] : R(0));
+      for (int m = 0; m < 5; ++m) amdq[m] = sf * wave[m * 3];
+      if (s[1] < R(0)) {
+        for (int m = 0; m < 5; ++m) amdq[m] += s[1] * wave[m * 3 + 1];
+        R rr = rho(qr), prr = pres(qr);
+        R s3 = qr[1] / rr + sqrt(gm * prr / rr);
+        R b[5];
+        for (int m = 0; m < 5; ++m) b[m] = qr[m] - wave[m * 3 + 2];
+        b[0] = rmax(b[0], rmin_);
+        R ke2 = R(0.5) * (b[1] * b[1] + b[2] * b[2] + b[3] * b[3]) / b[0];
+        R p2 = rmax((gm - R(1)) * (b[4] - ke2), pmin_);
+        R s2 = b[1] / b[0] + sqrt(gm * p2 / b[0]);
+        R sf2 = (s2 < R(0) && s3 > R(0)) ? s2 * (s3 - s[2]) / (s3 - s2) : (s[2] < R(0) ? s[2] : R(0));
+        for (int m = 0; m < 5; ++m) amdq[m] += sf2 * wave[m * 3 + 2];
+      }
+    }
+    for (int m = 0; m < 5; ++m) {
+      R t = R(0);
+      for (int w = 0; w < 3; ++w) t += s[w] * wave[m * 3 + w];
+      apdq[m] = t - amdq[m];
+    }
+  }
+  MXM_HD static void split(const R *s, const R *wave, R *amdq, R *apdq) {
+    for (int m = 0; m < 5; ++m) {
+      amdq[m] = R(0);
+      apdq[m] = R(0);
+      for (int w = 0; w < 3; ++w) {
+        if (s[w] < R(0))
+          amdq[m] += s[w] * wave[m * 3 + w];
+        else
+          apdq[m] += s[w] * wave[m * 3 + w];
+      }
+    }
+  }
+  // face-aligned flux (rl, rr already rotated); src/euler.mojo:612-683
+  MXM_HD void local_flux(const R *rl, const R *rr, R *rf) const {
+    if constexpr (FLUX == 0) {
+      R dl = rho(rl), dr = rho(rr), pl = pres(rl), pr = pres(rr);
+      R cl = sqrt(gamma * pl / dl), cr = sqrt(gamma * pr / dr);
+      R ul = rl[1] / dl, ur = rr[1] / dr;
+      R cm = rmax(rabs(ul) + cl, rabs(ur) + cr);
+      rf[0] = R(0.5) * (rl[1] + rr[1] + cm * (dl - dr));
+      rf[1] = R(0.5) * (rl[1] * ul + pl + rr[1] * ur + pr + cm * (rl[1] - rr[1]));
+      rf[2] = R(0.5) * (rl[2] * ul + rr[2] * ur + cm * (rl[2] - rr[2]));
+      rf[3] = R(0.5) * (rl[3] * ul + rr[3] * ur + cm * (rl[3] - rr[3]));
+      rf[4] = R(0.5) * ((rl[4] + pl) * ul + (rr[4] + pr) * ur + cm * (rl[4] - rr[4]));
+    } else {
+      R s[3], wave[15], amdq[5], apdq[5];
+      R pl = pres(rl), pr = pres(rr);
+      R ra[5];
+      roe_avg(rl, rr, pl, pr, ra);
+      if constexpr (FLUX == 1) {
+        // Roe, src/euler.mojo:296-361
+        const R g1 = gamma - R(1);
+        R d[5];
+        for (int m = 0; m < 5; ++m) d[m] = rr[m] - rl[m];
+        R u = ra[0], v = ra[1], w = ra[2], H = ra[3], a = ra[4];
+        R u2 = u * u + v * v + w * w;
+        R a4 = g1 / (a * a) * ((H - u2) * d[0] + u * d[1] + v * d[2] + w * d[3] - d[4]);
+        R a2 = d[2] - v * d[0], a3 = d[3] - w * d[0];
+        R a5 = (d[1] + (a - u) * d[0] - a * a4) / (R(2) * a);
+        R a1 = d[0] - a4 - a5;
+        wave[0] = a1, wave[3] = a1 * (u - a), wave[6] = a1 * v, wave[9] = a1 * w;
+        wave[12] = a1 * (H - u * a);
+        wave[1] = a4, wave[4] = a4 * u, wave[7] = a4 * v + a2, wave[10] = a4 * w + a3;
+        wave[13] = a4 * R(0.5) * u2 + a2 * v + a3 * w;
+        wave[2] = a5, wave[5] = a5 * (u + a), wave[8] = a5 * v, wave[11] = a5 * w;
+        wave[14] = a5 * (H + u * a);
+        s[0] = u - a, s[1] = u, s[2] = u + a;
+      } else {
+        R dl = rho(rl), dr = rho(rr);
+        R ul = rl[1] / dl, ur = rr[1] / dr;
+        s[0] = rmin(ul - sqrt(gamma * pl / dl), ra[0] - ra[4]);
+        s[2] = rmax(ur + sqrt(gamma * pr / dr), ra[0] + ra[4]);
+        if constexpr (FLUX == 2) {
+          // HLLE, src/euler.mojo:364-412
+          s[1] = R(0);
+          R fl[5], fr[5];
+          flux1(rl, fl);
+          flux1(rr, fr);
+          R den = s[0] - s[2];
+          for (int m = 0; m < 5; ++m) {
+            R qh = (fr[m] - fl[m] - s[2] * rr[m] + s[0] * rl[m]) / den;
+            wave[m * 3] = qh - rl[m];
+            wave[m * 3 + 1] = R(0);
+            wave[m * 3 + 2] = rr[m] - qh;
+          }
+        } else {
+          // HLLEC, src/euler.mojo:415-485
+          s[1] = (pr - pl + dl * ul * (s[0] - ul) - dr * ur * (s[2] - ur)) /
+                 (dl * (s[0] - ul) - dr * (s[2] - ur));
+          R lm = dl * (s[0] - ul) / (s[0] - s[1]);
+          R hl[5] = {lm, lm * s[1], lm * rl[2] / dl, lm * rl[3] / dl,
+                     lm * (rl[4] / dl + (s[1] - ul) * (s[1] + pl / (dl * (s[0] - ul))))};
+          R rm = dr * (s[2] - ur) / (s[2] - s[1]);
+          R hr[5] = {rm, rm * s[1], rm * rr[2] / dr, rm * rr[3] / dr,
+                     rm * (rr[4] / dr + (s[1] - ur) * (s[1] + pr / (dr * (s[2] - ur))))};
+          for (int m = 0; m < 5; ++m) {
+            wave[m * 3] = hl[m] - rl[m];
+            wave[m * 3 + 1] = hr[m] - hl[m];
+            wave[m * 3 + 2] = rr[m] - hr[m];
+          }
+        }
+      }
+      if (efix)
+        entropy_fixed(rl, rr, s, wave, amdq, apdq);
+      else
+        split(s, wave, amdq, apdq);
+      R fl[5], fr[5];
+      flux1(rl, fl);
+      flux1(rr, fr);
+      for (int m = 0; m < 5; ++m) rf[m] = R(0.5) * (fl[m] + fr[m]) - R(0.5) * (apdq[m] - amdq[m]);
+    }
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    Frame F = frame(n);
+    R rl[5], rr[5], rf[5];
+    to_local(F, ql, rl);
+    to_local(F, qr, rr);
+    local_flux(rl, rr, rf);
+    to_world(F, rf, f);
+  }
+  // ghost states, src/euler.mojo:705-730
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R g[5];
+    if (bc == BC_WALL) {
+      R mn = q[1] * n[0] + q[2] * n[1] + q[3] * n[2];
+      g[0] = q[0];
+      g[4] = q[4];
+      for (int d = 0; d < 3; ++d) g[1 + d] = q[1 + d] - R(2) * mn * n[d];
+    } else if (bc == BC_INFLOW) {
+      for (int c = 0; c < 5; ++c) g[c] = inflow[c];
+    } else {
+      for (int c = 0; c < 5; ++c) g[c] = q[c];
+    }
+    numerical_flux(q, g, n, f);
+  }
+  // gravity, src/euler.mojo:738-747
+  MXM_HD void source(const R *q, R *s) const {
+    s[0] = R(0);
+    s[1] = q[0] * g[0];
+    s[2] = q[0] * g[1];
+    s[3] = q[0] * g[2];
+    s[4] = q[1] * g[0] + q[2] * g[1] + q[3] * g[2];
+  }
+  // NaN-safe positivity rescue, src/euler.mojo:758-792
+  MXM_HD void limit(R *q) const {
+    const R e0 = pmin_ / (gamma - R(1));
+    if (!(q[0] > rmin_)) {
+      q[0] = rmin_;
+      q[1] = q[2] = q[3] = R(0);
+      q[4] = e0;
+      return;
+    }
+    R m2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (!(m2 >= R(0))) {
+      q[1] = q[2] = q[3] = R(0);
+      q[4] = e0;
+      return;
+    }
+    R ke = R(0.5) * m2 / q[0];
+    R p = (gamma - R(1)) * (q[4] - ke);
+    if (!(p > pmin_)) q[4] = ke + e0;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// 2D module physics.  Boundary faces build the module's ghost state and call
+// the same face flux; flux<2> is the nodal flux of the module's
+// vol_lift_combine_rk kernel.
+// ---------------------------------------------------------------------------
+
+// src/local_mesh_2d_gpu_euler.mojo; HLLC when HLLC == 1
+template <class R, int HLLC>
+struct Euler2D {
+  static constexpr int NC = 4;
+  static constexpr bool kSource = true, kLimit = false;
+  R gamma, rmin_, pmin_, gx, gy;
+  R inflow[4];
+  static Euler2D from(const double *p) {
+    Euler2D e;
+    e.gamma = R(p[0]), e.rmin_ = R(p[1]), e.pmin_ = R(p[2]), e.gx = R(p[4]), e.gy = R(p[5]);
+    for (int c = 0; c < 4; ++c) e.inflow[c] = R(p[6 + c]);
+    return e;
+  }
+  // primitive state with floors (volume and face kernels use the same)
+  MXM_HD void prim(const R *q, R &r, R &u, R &v, R &p) const {
+    r = q[0] < rmin_ ? rmin_ : q[0];
+    u = q[1] / r;
+    v = q[2] / r;
+    R ke = R(0.5) * r * (u * u + v * v);
+    p = (gamma - R(1)) * (q[3] - ke);
+    if (p < pmin_) p = pmin_;
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    R r, u, v, p;
+    prim(q, r, u, v, p);
+    F[0] = q[1], F[1] = q[1] * u + p, F[2] = q[1] * v, F[3] = u * (q[3] + p);
+    F[4] = q[2], F[5] = q[2] * u, F[6] = q[2] * v + p, F[7] = v * (q[3] + p);
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    const R nx = n[0], ny = n[1];
+    R rl, ul, vl, pl, rr, ur, vr, pr;
+    prim(ql, rl, ul, vl, pl);
+    prim(qr, rr, ur, vr, pr);
+    if constexpr (HLLC == 0) {
+      R FL[8], FR[8];
+      flux<2>(ql, FL);
+      flux<2>(qr, FR);
+      R sl = sqrt(ul * ul + vl * vl) + sqrt(gamma * pl / rl);
+      R sr = sqrt(ur * ur + vr * vr) + sqrt(gamma * pr / rr);
+      R al = sl > sr ? sl : sr;
+      for (int c = 0; c < 4; ++c)
+        f[c] = R(0.5) * ((FL[c] + FR[c]) * nx + (FL[4 + c] + FR[4 + c]) * ny) - R(0.5) * al * (qr[c] - ql[c]);
+    } else {
+      // src/local_mesh_2d_gpu_euler.mojo:466-545
+      R al = sqrt(gamma * pl / rl), ar = sqrt(gamma * pr / rr);
+      R unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
+      R FnL[4] = {ql[1] * nx + ql[2] * ny, (ql[1] * ul + pl) * nx + (ql[1] * vl) * ny,
+                  (ql[1] * vl) * nx + (ql[2] * vl + pl) * ny, (ul * (ql[3] + pl)) * nx + (vl * (ql[3] + pl)) * ny};
+      R FnR[4] = {qr[1] * nx + qr[2] * ny, (qr[1] * ur + pr) * nx + (qr[1] * vr) * ny,
+                  (qr[1] * vr) * nx + (qr[2] * vr + pr) * ny, (ur * (qr[3] + pr)) * nx + (vr * (qr[3] + pr)) * ny};
+      R SL = rmin(unl - al, unr - ar), SR = rmax(unl + al, unr + ar);
+      R ss = (pr - pl + rl * unl * (SL - unl) - rr * unr * (SR - unr)) / (rl * (SL - unl) - rr * (SR - unr));
+      if (SL >= R(0)) {
+        for (int c = 0; c < 4; ++c) f[c] = FnL[c];
+      } else if (SR <= R(0)) {
+        for (int c = 0; c < 4; ++c) f[c] = FnR[c];
+      } else {
+        const bool left = ss >= R(0);
+        const R *q = left ? ql : qr;
+        const R *Fn = left ? FnL : FnR;
+        R S = left ? SL : SR, d = left ? rl : rr, u = left ? ul : ur, v = left ? vl : vr;
+        R un = left ? unl : unr, p = left ? pl : pr;
+        R coef = (S - un) / (S - ss);
+        R rs = d * coef;
+        R us = u + (ss - un) * nx, vs = v + (ss - un) * ny;
+        R Es = q[3] / d + (ss - un) * (ss + p / (d * (S - un)));
+        R qs[4] = {rs, rs * us, rs * vs, rs * Es};
+        for (int c = 0; c < 4; ++c) f[c] = Fn[c] + S * (qs[c] - q[c]);
+      }
+    }
+  }
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R g[4];
+    if (bc == BC_WALL) {
+      R mn = q[1] * n[0] + q[2] * n[1];
+      g[0] = q[0], g[1] = q[1] - R(2) * mn * n[0], g[2] = q[2] - R(2) * mn * n[1], g[3] = q[3];
+    } else if (bc == BC_INFLOW) {
+      for (int c = 0; c < 4; ++c) g[c] = inflow[c];
+    } else {
+      for (int c = 0; c < 4; ++c) g[c] = q[c];
+    }
+    numerical_flux(q, g, n, f);
+  }
+  // gravity with the floored density, src/local_mesh_2d_gpu_euler.mojo:126-140
+  MXM_HD void source(const R *q, R *s) const {
+    R r = q[0] < rmin_ ? rmin_ : q[0];
+    s[0] = R(0), s[1] = r * gx, s[2] = r * gy, s[3] = q[1] * gx + q[2] * gy;
+  }
+  MXM_HD void limit(R *) const {}
+};
+
+// src/local_mesh_2d_gpu_sw.mojo; HLL when HLL == 1
+template <class R, int HLL>
+struct SW2D {
+  static constexpr int NC = 3;
+  static constexpr bool kSource = false, kLimit = false;
+  R grav, hmin;
+  R inflow[3];
+  static SW2D from(const double *p) { return {R(p[0]), R(p[1]), {R(p[3]), R(p[4]), R(p[5])}}; }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    R h = q[0] < hmin ? hmin : q[0];
+    R u = q[1] / h, v = q[2] / h, p = R(0.5) * grav * h * h;
+    F[0] = q[1], F[1] = q[1] * u + p, F[2] = q[1] * v;
+    F[3] = q[2], F[4] = q[2] * u, F[5] = q[2] * v + p;
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    const R nx = n[0], ny = n[1];
+    R FL[6], FR[6];
+    flux<2>(ql, FL);
+    flux<2>(qr, FR);
+    R hl = ql[0] < hmin ? hmin : ql[0], hr = qr[0] < hmin ? hmin : qr[0];
+    R ul = ql[1] / hl, vl = ql[2] / hl, ur = qr[1] / hr, vr = qr[2] / hr;
+    if constexpr (HLL == 0) {
+      R sl = sqrt(ul * ul + vl * vl) + sqrt(grav * hl), sr = sqrt(ur * ur + vr * vr) + sqrt(grav * hr);
+      R al = sl > sr ? sl : sr;
+      for (int c = 0; c < 3; ++c)
+        f[c] = R(0.5) * ((FL[c] + FR[c]) * nx + (FL[3 + c] + FR[3 + c]) * ny) - R(0.5) * al * (qr[c] - ql[c]);
+    } else {
+      R cl = sqrt(grav * hl), cr = sqrt(grav * hr);
+      R unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
+      R SL = rmin(unl - cl, unr - cr), SR = rmax(unl + cl, unr + cr);
+      R FnL[3], FnR[3];
+      for (int c = 0; c < 3; ++c) {
+        FnL[c] = FL[c] * nx + FL[3 + c] * ny;
+        FnR[c] = FR[c] * nx + FR[3 + c] * ny;
+      }
+      if (SL >= R(0)) {
+        for (int c = 0; c < 3; ++c) f[c] = FnL[c];
+      } else if (SR <= R(0)) {
+        for (int c = 0; c < 3; ++c) f[c] = FnR[c];
+      } else {
+        R inv = R(1) / (SR - SL);
+        for (int c = 0; c < 3; ++c) f[c] = (SR * FnL[c] - SL * FnR[c] + SL * SR * (qr[c] - ql[c])) * inv;
+      }
+    }
+  }
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R g[3];
+    if (bc == BC_WALL) {
+      R mn = q[1] * n[0] + q[2] * n[1];
+      g[0] = q[0], g[1] = q[1] - R(2) * mn * n[0], g[2] = q[2] - R(2) * mn * n[1];
+    } else if (bc == BC_INFLOW) {
+      for (int c = 0; c < 3; ++c) g[c] = inflow[c];
+    } else {
+      for (int c = 0; c < 3; ++c) g[c] = q[c];
+    }
+    numerical_flux(q, g, n, f);
+  }
+  MXM_HD void source(const R *, R *s) const { s[0] = s[1] = s[2] = R(0); }
+  MXM_HD void limit(R *) const {}
+};
+
+// src/local_mesh_2d_gpu_mhd_glm.mojo (GLM-MHD, Rusanov)
+template <class R>
+struct MHDGLM2D {
+  static constexpr int NC = 7;
+  static constexpr bool kSource = false, kLimit = false;
+  R gamma, rmin_, pmin_, ch;
+  R inflow[7];
+  static MHDGLM2D from(const double *p) {
+    MHDGLM2D e;
+    e.gamma = R(p[0]), e.rmin_ = R(p[1]), e.pmin_ = R(p[2]), e.ch = R(p[3]);
+    for (int c = 0; c < 7; ++c) e.inflow[c] = R(p[5 + c]);
+    return e;
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    R sp;
+    state(q, F, sp);
+  }
+  MXM_HD void state(const R *q, R *F, R &speed) const {
+    R r = q[0] < rmin_ ? rmin_ : q[0];
+    R u = q[1] / r, v = q[2] / r, Bx = q[3], By = q[4], E = q[5], psi = q[6];
+    R BB = Bx * Bx + By * By;
+    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2]) / r;
+    R p = (gamma - R(1)) * (E - ke - R(0.5) * BB);
+    if (p < pmin_) p = pmin_;
+    R ps = p + R(0.5) * BB, ch2 = ch * ch;
+    F[0] = q[1], F[1] = q[1] * u + ps - Bx * Bx, F[2] = q[1] * v - Bx * By, F[3] = psi;
+    F[4] = u * By - v * Bx, F[5] = (E + ps) * u - Bx * (u * Bx + v * By), F[6] = ch2 * Bx;
+    F[7] = q[2], F[8] = q[2] * u - By * Bx, F[9] = q[2] * v + ps - By * By, F[10] = v * Bx - u * By;
+    F[11] = psi, F[12] = (E + ps) * v - By * (u * Bx + v * By), F[13] = ch2 * By;
+    R cs2 = gamma * p / r, ca2 = BB / r, s = cs2 + ca2;
+    R disc = s * s - R(4) * cs2 * (Bx * Bx) / r;
+    if (disc < R(0)) disc = R(0);
+    speed = rmax(sqrt(u * u + v * v) + sqrt(R(0.5) * (s + sqrt(disc))), ch);
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R FL[14], FR[14], sl, sr;
+    state(ql, FL, sl);
+    state(qr, FR, sr);
+    R al = sl > sr ? sl : sr;
+    for (int c = 0; c < 7; ++c)
+      f[c] = R(0.5) * ((FL[c] + FR[c]) * n[0] + (FL[7 + c] + FR[7 + c]) * n[1]) - R(0.5) * al * (qr[c] - ql[c]);
+  }
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R g[7];
+    if (bc == BC_WALL) {
+      R mn = q[1] * n[0] + q[2] * n[1], Bn = q[3] * n[0] + q[4] * n[1];
+      g[0] = q[0], g[1] = q[1] - R(2) * mn * n[0], g[2] = q[2] - R(2) * mn * n[1];
+      g[3] = q[3] - R(2) * Bn * n[0], g[4] = q[4] - R(2) * Bn * n[1], g[5] = q[5], g[6] = -q[6];
+    } else if (bc == BC_INFLOW) {
+      for (int c = 0; c < 7; ++c) g[c] = inflow[c];
+    } else {
+      for (int c = 0; c < 7; ++c) g[c] = q[c];
+    }
+    numerical_flux(q, g, n, f);
+  }
+  MXM_HD void source(const R *, R *s) const {
+    for (int c = 0; c < 7; ++c) s[c] = R(0);
+  }
+  MXM_HD void limit(R *) const {}
+};
+
+// src/local_mesh_2d_gpu_maxwell.mojo (Rusanov with alpha = c, PEC walls)
+template <class R>
+struct Maxwell2D {
+  static constexpr int NC = 6;
+  static constexpr bool kSource = true, kLimit = false;
+  R c, J[3], M[3], inflow[6];
+  static Maxwell2D from(const double *p) {
+    Maxwell2D m;
+    m.c = R(p[0]);
+    for (int d = 0; d < 3; ++d) m.J[d] = R(p[1 + d]), m.M[d] = R(p[4 + d]);
+    for (int k = 0; k < 6; ++k) m.inflow[k] = R(p[7 + k]);
+    return m;
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    R c2 = c * c;
+    F[0] = R(0), F[1] = c2 * q[5], F[2] = -c2 * q[4], F[3] = R(0), F[4] = -q[2], F[5] = q[1];
+    F[6] = -c2 * q[5], F[7] = R(0), F[8] = c2 * q[3], F[9] = q[2], F[10] = R(0), F[11] = -q[0];
+  }
+  MXM_HD static void normal_flux(const R *q, R c2, R nx, R ny, R *F) {
+    F[0] = c2 * (-q[5] * ny), F[1] = c2 * (q[5] * nx), F[2] = c2 * (q[3] * ny - q[4] * nx);
+    F[3] = ny * q[2], F[4] = -nx * q[2], F[5] = nx * q[1] - ny * q[0];
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R c2 = c * c, FL[6], FR[6];
+    normal_flux(ql, c2, n[0], n[1], FL);
+    normal_flux(qr, c2, n[0], n[1], FR);
+    for (int k = 0; k < 6; ++k) f[k] = R(0.5) * (FL[k] + FR[k]) - R(0.5) * c * (qr[k] - ql[k]);
+  }
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R g[6];
+    if (bc == BC_WALL) {
+      R En = q[0] * n[0] + q[1] * n[1], Bn = q[3] * n[0] + q[4] * n[1];
+      g[0] = R(2) * En * n[0] - q[0], g[1] = R(2) * En * n[1] - q[1], g[2] = -q[2];
+      g[3] = q[3] - R(2) * Bn * n[0], g[4] = q[4] - R(2) * Bn * n[1], g[5] = q[5];
+    } else if (bc == BC_INFLOW) {
+      for (int k = 0; k < 6; ++k) g[k] = inflow[k];
+    } else {
+      for (int k = 0; k < 6; ++k) g[k] = q[k];
+    }
+    numerical_flux(q, g, n, f);
+  }
+  // uniform J / M sources, src/local_mesh_2d_gpu_maxwell.mojo:133-142
+  MXM_HD void source(const R *, R *s) const {
+    R c2 = c * c;
+    for (int d = 0; d < 3; ++d) s[d] = -(c2 * J[d]), s[3 + d] = -M[d];
+  }
+  MXM_HD void limit(R *) const {}
+};
+
+// src/local_mesh_2d_gpu_advection.mojo (scalar upwind)
+template <class R>
+struct Advection2D {
+  static constexpr int NC = 1;
+  static constexpr bool kSource = false, kLimit = false;
+  R vx, vy, inflow;
+  static Advection2D from(const double *p) { return {R(p[0]), R(p[1]), R(p[2])}; }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    F[0] = vx * q[0];
+    F[1] = vy * q[0];
+  }
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R vn = vx * n[0] + vy * n[1];
+    f[0] = vn >= R(0) ? vn * ql[0] : vn * qr[0];
+  }
+  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
+    R vn = vx * n[0] + vy * n[1];
+    f[0] = vn >= R(0) ? vn * q[0] : (bc == BC_INFLOW ? vn * inflow : R(0));
+  }
+  MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
+  MXM_HD void limit(R *) const {}
+};
+
+}  // namespace mxm

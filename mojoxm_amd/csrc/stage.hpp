@@ -1,0 +1,233 @@
+// stage.hpp -- the fused DG stage kernel for gfx950 and its launcher.
+//
+// One launch computes, for every element, the weak-form right-hand side
+//   rhs_i = sum_k sum_j D_k[i][j] (invJ F(q_j))_k
+//           - inv_vol * sum_f sum_r Lift_f[i][r] s_f area_f F*_{f,r}
+//           + S(q_i)
+// and the Shu-Osher combine q_out = a q_a + b q_b + c dt rhs followed by the
+// physics' positivity fix -- rk_stage_kernel, src/solver.mojo:161-337, and
+// (for 2D) the face-flux + vol_lift_combine_rk kernel pair of
+// src/local_mesh_2d_gpu_<phys>.mojo fused into one pass.
+//
+// Work decomposition (256 threads = 4 wave64 per block, EPB elements per
+// block):
+//   1. the block's contiguous q_in tile, its geometry and (when small) the
+//      reference operators are staged in LDS with coalesced loads;
+//   2. per (element, node): the nodal flux, mapped to reference
+//      directions by invJ, into LDS;
+//      per (element, face, slot): the numerical or boundary flux, signed
+//      and scaled by the face area, into LDS -- the partner state comes
+//      from LDS when the neighbour is in the same tile, from global memory
+//      otherwise; side-0 is always the left state as in the reference;
+//   3. per (element, node): the D and Lift contractions over LDS, source,
+//      RK combine, limit_state, store.
+// Blocks are mapped to tiles so that each of the 8 XCDs walks a contiguous
+// range of tiles: face neighbours then mostly hit the XCD's own L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "physics.hpp"
+
+namespace mxm {
+
+template <class R>
+struct StageArgs {
+  const R *q_in, *q_a, *q_b;
+  R *q_out;
+  R a, b, cdt;
+  int K;
+  const R *invJ;     // [K][ND*ND] rows = reference directions
+  const R *inv_vol;  // [K] 1/(6V) or 1/(2A)
+  const int4 *conn;  // [K][NF] {partner, face id, side | bc<<1 | perm<<4, 0}
+  const R *fnormal;  // [Nf][ND]
+  const R *farea;    // [Nf]
+  const int *perm;   // [nperm][NFP] partner node per reference face slot
+  const R *Dt;       // [ND][NP(j)][NP(i)]  transposed D_ref
+  const R *Lt;       // [NF][NFP(r)][NP(i)] transposed Lift_ref
+  const int *fmap;   // [NF][NFP] element node of each reference face slot
+  int post_comp;     // component scaled after the combine (GLM psi), -1 none
+  R post_scale;
+};
+
+template <int ND, int P>
+struct Shape {
+  static constexpr int NP = ND == 3 ? (P + 1) * (P + 2) * (P + 3) / 6 : (P + 1) * (P + 2) / 2;
+  static constexpr int NFP = ND == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
+  static constexpr int NF = ND + 1;
+  static constexpr int BLOCK = 256;
+  static constexpr int EPB = BLOCK / NP;
+};
+
+// Bijective map from the hardware block index (dispatched round-robin over
+// the 8 XCDs) to a tile index such that XCD x owns a contiguous tile range.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = bid & 7, loc = bid >> 3;
+  return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
+}
+
+template <class Phys, int ND, int P, class R>
+__global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys) {
+  using S = Shape<ND, P>;
+  constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
+  constexpr int NOPS = ND * NP * NP + NF * NP * NFP;
+  constexpr bool kOpsLDS = NOPS * (int)sizeof(R) <= 24 * 1024;
+  __shared__ R sQ[EPB * NP * NC];
+  __shared__ R sF[EPB * ND * NP * NC];      // [el][k][j][c]
+  __shared__ R sS[EPB * NF * NFP * NC];     // [el][f][r][c]
+  __shared__ R sJ[EPB * ND * ND + EPB];     // invJ then inv_vol
+  __shared__ R sOp[kOpsLDS ? NOPS : 1];
+  __shared__ int sMap[NF * NFP];
+
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int e0 = tile * EPB;
+  const int nE = min(EPB, A.K - e0);
+  if (nE <= 0) return;
+
+  // ---- 1. stage the tile ------------------------------------------------
+  {
+    const R *src = A.q_in + (size_t)e0 * NP * NC;
+    const int n = nE * NP * NC;
+    for (int t = tid; t < n; t += S::BLOCK) sQ[t] = src[t];
+    const R *gJ = A.invJ + (size_t)e0 * ND * ND;
+    for (int t = tid; t < nE * ND * ND; t += S::BLOCK) sJ[t] = gJ[t];
+    for (int t = tid; t < nE; t += S::BLOCK) sJ[EPB * ND * ND + t] = A.inv_vol[e0 + t];
+    for (int t = tid; t < NF * NFP; t += S::BLOCK) sMap[t] = A.fmap[t];
+    if constexpr (kOpsLDS) {
+      for (int t = tid; t < ND * NP * NP; t += S::BLOCK) sOp[t] = A.Dt[t];
+      for (int t = tid; t < NF * NP * NFP; t += S::BLOCK) sOp[ND * NP * NP + t] = A.Lt[t];
+    }
+  }
+  __syncthreads();
+  const R *Dop = kOpsLDS ? sOp : A.Dt;
+  const R *Lop = kOpsLDS ? sOp + ND * NP * NP : A.Lt;
+
+  // ---- 2a. contravariant nodal fluxes ------------------------------------
+  for (int t = tid; t < nE * NP; t += S::BLOCK) {
+    const int el = t / NP, j = t - el * NP;
+    R F[ND * NC];
+    phys.template flux<ND>(&sQ[(el * NP + j) * NC], F);
+    const R *iJ = &sJ[el * ND * ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R v = R(0);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * F[d * NC + c];
+        sF[((el * ND + k) * NP + j) * NC + c] = v;
+      }
+  }
+  // ---- 2b. face fluxes (each element evaluates its own faces) ------------
+  for (int t = tid; t < nE * NF * NFP; t += S::BLOCK) {
+    const int el = t / (NF * NFP), rem = t - el * NF * NFP;
+    const int lf = rem / NFP, r = rem - lf * NFP;
+    const int e = e0 + el;
+    const int4 cn = A.conn[(size_t)e * NF + lf];
+    const int side = cn.z & 1, bc = (cn.z >> 1) & 7, pid = cn.z >> 4;
+    const R *qme = &sQ[(el * NP + sMap[lf * NFP + r]) * NC];
+    R n[3];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) n[d] = A.fnormal[(size_t)cn.y * ND + d];
+    if constexpr (ND == 2) n[2] = R(0);
+    const R area = A.farea[cn.y];
+    R qp[NC];
+    const bool need_partner = !(side == 0 && bc != 0);
+    if (need_partner) {
+      const int pe = cn.x, pn = A.perm[pid * NFP + r];
+      const int pl = pe - e0;
+      if (pl >= 0 && pl < nE) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) qp[c] = sQ[(pl * NP + pn) * NC + c];
+      } else {
+        const R *g = A.q_in + ((size_t)pe * NP + pn) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) qp[c] = g[c];
+      }
+    }
+    R Fs[NC];
+    if (side == 0) {
+      if (bc)
+        phys.boundary_flux(qme, bc, n, Fs);
+      else
+        phys.numerical_flux(qme, qp, n, Fs);
+    } else {
+      if (bc)
+        phys.boundary_flux(qp, bc, n, Fs);
+      else
+        phys.numerical_flux(qp, qme, n, Fs);
+    }
+    const R s = side == 0 ? area : -area;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sS[((el * NF + lf) * NFP + r) * NC + c] = s * Fs[c];
+  }
+  __syncthreads();
+
+  // ---- 3. contractions, source, RK combine, limit, store ----------------
+  for (int t = tid; t < nE * NP; t += S::BLOCK) {
+    const int el = t / NP, i = t - el * NP;
+    R vol[NC], fac[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) vol[c] = fac[c] = R(0);
+    for (int k = 0; k < ND; ++k) {
+      const R *Fk = &sF[(el * ND + k) * NP * NC];
+      const R *Dk = Dop + k * NP * NP;
+#pragma unroll 4
+      for (int j = 0; j < NP; ++j) {
+        const R dv = Dk[j * NP + i];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) vol[c] += dv * Fk[j * NC + c];
+      }
+    }
+    const R *Se = &sS[el * NF * NFP * NC];
+#pragma unroll 4
+    for (int fr = 0; fr < NF * NFP; ++fr) {
+      const R lv = Lop[fr * NP + i];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) fac[c] += lv * Se[fr * NC + c];
+    }
+    const R iv = sJ[EPB * ND * ND + el];
+    const R *qi = &sQ[(el * NP + i) * NC];
+    R src[NC];
+    if constexpr (Phys::kSource)
+      phys.source(qi, src);
+    else
+#pragma unroll
+      for (int c = 0; c < NC; ++c) src[c] = R(0);
+    const size_t g = ((size_t)(e0 + el) * NP + i) * NC;
+    R out[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const R rhs = vol[c] - iv * fac[c] + src[c];
+      out[c] = A.a * A.q_a[g + c] + A.b * A.q_b[g + c] + A.cdt * rhs;
+    }
+    if constexpr (Phys::kLimit) phys.limit(out);
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      if (c == A.post_comp) out[c] *= A.post_scale;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) A.q_out[g + c] = out[c];
+  }
+}
+
+template <class Phys, int ND, int P>
+hipError_t launch_stage(const StageArgs<double> &a, const double *prm, hipStream_t s) {
+  using S = Shape<ND, P>;
+  const Phys ph = Phys::from(prm);
+  const int nb = (a.K + S::EPB - 1) / S::EPB;
+  if (nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, double>), dim3(nb), dim3(S::BLOCK), 0, s, a, ph);
+  return hipGetLastError();
+}
+
+using StageLaunch = hipError_t (*)(const StageArgs<double> &, const double *, hipStream_t);
+
+// Lookups implemented in the per-family translation units; nullptr when
+// the (physics, variant, P) combination is not compiled.
+StageLaunch find_stage_3d(int phys, int variant, int P);
+StageLaunch find_stage_2d_trait(int phys, int variant, int P);
+StageLaunch find_stage_2d_module(int phys, int variant, int P);
+
+}  // namespace mxm

@@ -1,0 +1,63 @@
+"""Fixed-dt SSPRK3 loop with frame clamping (run_ssprk3_loop,
+src/time_integrator.mojo:61-116) and the drivers' a-priori dt rule
+(examples/advection_gaussian.mojo:126-131)."""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+
+@dataclass
+class TimeLoopResult:
+    total_steps: int
+    wall_sec: float
+    step_loop_sec: float
+    final_sync_sec: float
+    t_final: float
+
+
+def apriori_dt(cfl: float, h: float, wave: float, P: int) -> float:
+    """dt = CFL h / (wave (2P+1))."""
+    return cfl * h / (wave * (2 * P + 1))
+
+
+def fixed_steps(t_final: float, dt_est: float):
+    """steps = ceil(T/dt), dt = T/steps (the config convention)."""
+    n = max(1, int(math.ceil(t_final / dt_est - 1e-12)))
+    return n, t_final / n
+
+
+def run_ssprk3_loop(solver, dt: float, t_final: float, num_frames: int = 1, on_frame=None) -> TimeLoopResult:
+    """Step to t_final, clamping each step to land on frame boundaries and on
+    t_final; on_frame(solver, t) is called at each frame boundary."""
+    t = 0.0
+    frame_dt = t_final / num_frames
+    next_frame = frame_dt
+    frame_id = 1
+    steps = 0
+    tloop = 0.0
+    w0 = time.perf_counter()
+    while t < t_final:
+        step_dt = dt
+        if t + step_dt > next_frame:
+            step_dt = next_frame - t
+        if step_dt <= 0.0:
+            step_dt = dt
+        if t + step_dt > t_final:
+            step_dt = t_final - t
+        s0 = time.perf_counter()
+        solver.step_ssprk3(step_dt)
+        tloop += time.perf_counter() - s0
+        t += step_dt
+        steps += 1
+        if t >= next_frame - 1e-12 and frame_id < num_frames + 1:
+            if on_frame is not None:
+                solver.synchronize()
+                on_frame(solver, t)
+            frame_id += 1
+            next_frame += frame_dt
+    s0 = time.perf_counter()
+    solver.synchronize()
+    s1 = time.perf_counter()
+    return TimeLoopResult(steps, s1 - w0, tloop, s1 - s0, t)
