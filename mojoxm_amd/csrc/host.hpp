@@ -60,4 +60,18 @@ struct Connectivity {
 // a face slot is its own reference-face node) and compresses them.
 Connectivity build_connectivity(const HostMesh &m, const RefElem &re);
 
+// Face-flux work lists of the stage kernel.  Elements are processed in
+// tiles of `epb` consecutive elements; within a tile, a face whose two
+// sides both lie in the tile is evaluated once (by its side-0 element),
+// which also writes the negated flux into the partner's slots.  For tile T
+// the jobs are jobs[job_off[T] .. job_off[T+1]), each packed as
+//   el | lf << 7 | (partner_local + 1 or 0) << 9 | partner_lf << 16 | pslot << 18
+// and pslot_tab[pslot*NFP + r] is the partner's reference face slot that
+// meets slot r.
+struct FaceJobs {
+  std::vector<int32_t> job_off, jobs, pslot_tab;
+  int npslot = 0;
+};
+FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivity &c, int epb);
+
 }  // namespace mxm

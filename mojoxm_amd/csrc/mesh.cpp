@@ -20,6 +20,7 @@
 //                                     src/local_mesh_2d.mojo:138-320
 //   the BC overlay numbering          src/local_mesh.mojo:946-1187,
 //                                     src/local_mesh_2d.mojo:331-395
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <stdexcept>
@@ -473,6 +474,79 @@ Connectivity build_connectivity(const HostMesh &m, const RefElem &re) {
   c.nperm = (int)pats.size();
   for (auto &p : pats) c.perm_tab.insert(c.perm_tab.end(), p.begin(), p.end());
   return c;
+}
+
+FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivity &c, int epb) {
+  const int NF = m.NF, NFP = m.NFP;
+  if (epb < 1 || epb > 126) throw std::runtime_error("jobs: tile size out of range");
+  FaceJobs J;
+  const int ntiles = (m.K + epb - 1) / epb;
+  J.job_off.assign(ntiles + 1, 0);
+  std::vector<std::vector<int>> pats;
+  std::vector<int> pat(NFP);
+  for (int T = 0; T < ntiles; ++T) {
+    const int e0 = T * epb, nE = std::min(epb, m.K - e0);
+    for (int el = 0; el < nE; ++el)
+      for (int lf = 0; lf < NF; ++lf) {
+        const int e = e0 + el;
+        const int32_t *cn = &c.conn[((size_t)e * NF + lf) * 4];
+        const int partner = cn[0], fid = cn[1], side = cn[2] & 1, bc = (cn[2] >> 1) & 7, pid = cn[2] >> 4;
+        const int pl = partner - e0;
+        // A pair needs both elements to reference the face: after a "-"
+        // boundary overlay on an axis whose "+" side stays periodic, the
+        // old periodic face keeps its side-0 element only.
+        int plf = -1;
+        if (bc == 0 && partner != e && pl >= 0 && pl < nE)
+          for (int q = 0; q < NF; ++q)
+            if (m.elem_faces[(size_t)partner * NF + q] == fid &&
+                m.elem_face_side[(size_t)partner * NF + q] == 1 - side)
+              plf = q;
+        const bool pair = plf >= 0;
+        if (pair && side == 1) continue;  // evaluated by the side-0 element
+        int packed = el | (lf << 7);
+        if (pair) {
+          for (int r = 0; r < NFP; ++r) {
+            const int pn = c.perm_tab[(size_t)pid * NFP + r];
+            int rs = -1;
+            for (int q = 0; q < NFP; ++q)
+              if (re.fmap[plf * NFP + q] == pn) rs = q;
+            if (rs < 0) throw std::runtime_error("jobs: partner slot not found");
+            pat[r] = rs;
+          }
+          int id = -1;
+          for (size_t p = 0; p < pats.size(); ++p)
+            if (pats[p] == pat) id = (int)p;
+          if (id < 0) {
+            id = (int)pats.size();
+            pats.push_back(pat);
+          }
+          if (id >= (1 << 13)) throw std::runtime_error("jobs: too many slot patterns");
+          packed |= ((pl + 1) << 9) | (plf << 16) | (id << 18);
+        }
+        J.jobs.push_back(packed);
+      }
+    J.job_off[T + 1] = (int)J.jobs.size();
+  }
+  J.npslot = (int)pats.size();
+  // Every (element, face, slot) of a tile must be written exactly once.
+  std::vector<int> hits((size_t)epb * NF * NFP);
+  for (int T = 0; T < ntiles; ++T) {
+    const int nE = std::min(epb, m.K - T * epb);
+    std::fill(hits.begin(), hits.end(), 0);
+    for (int j = J.job_off[T]; j < J.job_off[T + 1]; ++j) {
+      const int job = J.jobs[j], el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
+      if (el >= nE || (pin && pin - 1 >= nE)) throw std::runtime_error("jobs: element outside its tile");
+      for (int r = 0; r < NFP; ++r) {
+        ++hits[((size_t)el * NF + lf) * NFP + r];
+        if (pin) ++hits[((size_t)(pin - 1) * NF + ((job >> 16) & 3)) * NFP + pats[job >> 18][r]];
+      }
+    }
+    for (int k = 0; k < nE * NF * NFP; ++k)
+      if (hits[k] != 1) throw std::runtime_error("jobs: face slot not covered exactly once");
+  }
+  for (auto &p : pats) J.pslot_tab.insert(J.pslot_tab.end(), p.begin(), p.end());
+  if (J.pslot_tab.empty()) J.pslot_tab.assign(NFP, 0);
+  return J;
 }
 
 }  // namespace mxm

@@ -83,28 +83,42 @@ struct Euler {
     R p = (gamma - R(1)) * (q[4] - ke);
     return p > pmin_ ? p : pmin_;
   }
+  // Floored primitives (src/euler.mojo:44-56) from one reciprocal of the
+  // density: FP64 division is a long instruction sequence on CDNA, so the
+  // reference's repeated m/rho become multiplications by 1/rho.
+  struct Prim {
+    R r, ir, u[3], p;
+  };
+  MXM_HD Prim prim(const R *q) const {
+    Prim s;
+    s.r = q[0] > rmin_ ? q[0] : rmin_;
+    s.ir = R(1) / s.r;
+    for (int d = 0; d < 3; ++d) s.u[d] = q[1 + d] * s.ir;
+    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * s.ir;
+    R p = (gamma - R(1)) * (q[4] - ke);
+    s.p = p > pmin_ ? p : pmin_;
+    return s;
+  }
   template <int ND>
   MXM_HD void flux(const R *q, R *F) const {
-    R r = rho(q), p = pres(q);
-    R u[3] = {q[1] / r, q[2] / r, q[3] / r};
+    const Prim s = prim(q);
     for (int d = 0; d < ND; ++d) {
       R *Fd = F + d * 5;
       Fd[0] = q[1 + d];
-      Fd[1] = q[1 + d] * u[0];
-      Fd[2] = q[1 + d] * u[1];
-      Fd[3] = q[1 + d] * u[2];
-      Fd[1 + d] += p;
-      Fd[4] = u[d] * (q[4] + p);
+      Fd[1] = q[1 + d] * s.u[0];
+      Fd[2] = q[1 + d] * s.u[1];
+      Fd[3] = q[1 + d] * s.u[2];
+      Fd[1 + d] += s.p;
+      Fd[4] = s.u[d] * (q[4] + s.p);
     }
   }
   // x-direction flux of a face-aligned state
-  MXM_HD void flux1(const R *q, R *f) const {
-    R r = rho(q), p = pres(q), u = q[1] / r;
+  MXM_HD static void flux1(const R *q, const Prim &s, R *f) {
     f[0] = q[1];
-    f[1] = q[1] * u + p;
-    f[2] = q[2] * u;
-    f[3] = q[3] * u;
-    f[4] = u * (q[4] + p);
+    f[1] = q[1] * s.u[0] + s.p;
+    f[2] = q[2] * s.u[0];
+    f[3] = q[3] * s.u[0];
+    f[4] = s.u[0] * (q[4] + s.p);
   }
   struct Frame {
     R n[3], t[3], b[3];
@@ -119,10 +133,10 @@ struct Euler {
       a[0] += R(1);
     R pr = n[0] * a[0] + n[1] * a[1] + n[2] * a[2];
     for (int d = 0; d < 3; ++d) a[d] -= n[d] * pr;
-    R mag = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    const R imag = R(1) / sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
     for (int d = 0; d < 3; ++d) {
       F.n[d] = n[d];
-      F.t[d] = a[d] / mag;
+      F.t[d] = a[d] * imag;
     }
     F.b[0] = n[1] * F.t[2] - n[2] * F.t[1];
     F.b[1] = n[2] * F.t[0] - n[0] * F.t[2];
@@ -141,13 +155,12 @@ struct Euler {
     q[4] = r[4];
     for (int d = 0; d < 3; ++d) q[1 + d] = F.n[d] * r[1] + F.t[d] * r[2] + F.b[d] * r[3];
   }
-  // Roe averages, src/euler.mojo:161-176 (returns u, v, w, H, a)
-  MXM_HD void roe_avg(const R *ql, const R *qr, R pl, R pr, R *o) const {
-    R sl = sqrt(rho(ql)), sr = sqrt(rho(qr)), s = sl + sr;
-    o[0] = (ql[1] / sl + qr[1] / sr) / s;
-    o[1] = (ql[2] / sl + qr[2] / sr) / s;
-    o[2] = (ql[3] / sl + qr[3] / sr) / s;
-    o[3] = ((ql[4] + pl) / sl + (qr[4] + pr) / sr) / s;
+  // Roe averages, src/euler.mojo:161-176 (returns u, v, w, H, a):
+  // m/sqrt(rho) = sqrt(rho) u, so every term shares one reciprocal.
+  MXM_HD void roe_avg(const R *ql, const R *qr, const Prim &L, const Prim &Rt, R *o) const {
+    const R sl = sqrt(L.r), sr = sqrt(Rt.r), is = R(1) / (sl + sr);
+    for (int d = 0; d < 3; ++d) o[d] = (sl * L.u[d] + sr * Rt.u[d]) * is;
+    o[3] = (sl * ((ql[4] + L.p) * L.ir) + sr * ((qr[4] + Rt.p) * Rt.ir)) * is;
     R u2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
     o[4] = sqrt(rmax((gamma - R(1)) * (o[3] - R(0.5) * u2), R(0)));
   }
@@ -201,10 +214,10 @@ struct Euler {
   }
   // face-aligned flux (rl, rr already rotated); src/euler.mojo:612-683
   MXM_HD void local_flux(const R *rl, const R *rr, R *rf) const {
+    const Prim L = prim(rl), Rt = prim(rr);
+    const R dl = L.r, dr = Rt.r, pl = L.p, pr = Rt.p, ul = L.u[0], ur = Rt.u[0];
     if constexpr (FLUX == 0) {
-      R dl = rho(rl), dr = rho(rr), pl = pres(rl), pr = pres(rr);
-      R cl = sqrt(gamma * pl / dl), cr = sqrt(gamma * pr / dr);
-      R ul = rl[1] / dl, ur = rr[1] / dr;
+      R cl = sqrt(gamma * pl * L.ir), cr = sqrt(gamma * pr * Rt.ir);
       R cm = rmax(rabs(ul) + cl, rabs(ur) + cr);
       rf[0] = R(0.5) * (rl[1] + rr[1] + cm * (dl - dr));
       rf[1] = R(0.5) * (rl[1] * ul + pl + rr[1] * ur + pr + cm * (rl[1] - rr[1]));
@@ -213,9 +226,8 @@ struct Euler {
       rf[4] = R(0.5) * ((rl[4] + pl) * ul + (rr[4] + pr) * ur + cm * (rl[4] - rr[4]));
     } else {
       R s[3], wave[15], amdq[5], apdq[5];
-      R pl = pres(rl), pr = pres(rr);
       R ra[5];
-      roe_avg(rl, rr, pl, pr, ra);
+      roe_avg(rl, rr, L, Rt, ra);
       if constexpr (FLUX == 1) {
         // Roe, src/euler.mojo:296-361
         const R g1 = gamma - R(1);
@@ -223,9 +235,10 @@ struct Euler {
         for (int m = 0; m < 5; ++m) d[m] = rr[m] - rl[m];
         R u = ra[0], v = ra[1], w = ra[2], H = ra[3], a = ra[4];
         R u2 = u * u + v * v + w * w;
-        R a4 = g1 / (a * a) * ((H - u2) * d[0] + u * d[1] + v * d[2] + w * d[3] - d[4]);
+        const R ia = R(1) / a;
+        R a4 = g1 * (ia * ia) * ((H - u2) * d[0] + u * d[1] + v * d[2] + w * d[3] - d[4]);
         R a2 = d[2] - v * d[0], a3 = d[3] - w * d[0];
-        R a5 = (d[1] + (a - u) * d[0] - a * a4) / (R(2) * a);
+        R a5 = (d[1] + (a - u) * d[0] - a * a4) * (R(0.5) * ia);
         R a1 = d[0] - a4 - a5;
         wave[0] = a1, wave[3] = a1 * (u - a), wave[6] = a1 * v, wave[9] = a1 * w;
         wave[12] = a1 * (H - u * a);
@@ -235,33 +248,30 @@ struct Euler {
         wave[14] = a5 * (H + u * a);
         s[0] = u - a, s[1] = u, s[2] = u + a;
       } else {
-        R dl = rho(rl), dr = rho(rr);
-        R ul = rl[1] / dl, ur = rr[1] / dr;
-        s[0] = rmin(ul - sqrt(gamma * pl / dl), ra[0] - ra[4]);
-        s[2] = rmax(ur + sqrt(gamma * pr / dr), ra[0] + ra[4]);
+        s[0] = rmin(ul - sqrt(gamma * pl * L.ir), ra[0] - ra[4]);
+        s[2] = rmax(ur + sqrt(gamma * pr * Rt.ir), ra[0] + ra[4]);
         if constexpr (FLUX == 2) {
           // HLLE, src/euler.mojo:364-412
           s[1] = R(0);
           R fl[5], fr[5];
-          flux1(rl, fl);
-          flux1(rr, fr);
-          R den = s[0] - s[2];
+          flux1(rl, L, fl);
+          flux1(rr, Rt, fr);
+          const R iden = R(1) / (s[0] - s[2]);
           for (int m = 0; m < 5; ++m) {
-            R qh = (fr[m] - fl[m] - s[2] * rr[m] + s[0] * rl[m]) / den;
+            R qh = (fr[m] - fl[m] - s[2] * rr[m] + s[0] * rl[m]) * iden;
             wave[m * 3] = qh - rl[m];
             wave[m * 3 + 1] = R(0);
             wave[m * 3 + 2] = rr[m] - qh;
           }
         } else {
           // HLLEC, src/euler.mojo:415-485
-          s[1] = (pr - pl + dl * ul * (s[0] - ul) - dr * ur * (s[2] - ur)) /
-                 (dl * (s[0] - ul) - dr * (s[2] - ur));
-          R lm = dl * (s[0] - ul) / (s[0] - s[1]);
-          R hl[5] = {lm, lm * s[1], lm * rl[2] / dl, lm * rl[3] / dl,
-                     lm * (rl[4] / dl + (s[1] - ul) * (s[1] + pl / (dl * (s[0] - ul))))};
-          R rm = dr * (s[2] - ur) / (s[2] - s[1]);
-          R hr[5] = {rm, rm * s[1], rm * rr[2] / dr, rm * rr[3] / dr,
-                     rm * (rr[4] / dr + (s[1] - ur) * (s[1] + pr / (dr * (s[2] - ur))))};
+          const R Al = dl * (s[0] - ul), Ar = dr * (s[2] - ur);
+          s[1] = (pr - pl + Al * ul - Ar * ur) / (Al - Ar);
+          const R lm = Al / (s[0] - s[1]), rm = Ar / (s[2] - s[1]);
+          R hl[5] = {lm, lm * s[1], lm * L.u[1], lm * L.u[2],
+                     lm * (rl[4] * L.ir + (s[1] - ul) * (s[1] + pl / Al))};
+          R hr[5] = {rm, rm * s[1], rm * Rt.u[1], rm * Rt.u[2],
+                     rm * (rr[4] * Rt.ir + (s[1] - ur) * (s[1] + pr / Ar))};
           for (int m = 0; m < 5; ++m) {
             wave[m * 3] = hl[m] - rl[m];
             wave[m * 3 + 1] = hr[m] - hl[m];
@@ -274,8 +284,8 @@ struct Euler {
       else
         split(s, wave, amdq, apdq);
       R fl[5], fr[5];
-      flux1(rl, fl);
-      flux1(rr, fr);
+      flux1(rl, L, fl);
+      flux1(rr, Rt, fr);
       for (int m = 0; m < 5; ++m) rf[m] = R(0.5) * (fl[m] + fr[m]) - R(0.5) * (apdq[m] - amdq[m]);
     }
   }

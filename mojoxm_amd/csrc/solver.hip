@@ -86,7 +86,8 @@ struct Solver {
   DevBuf<double> q, q1, q2, mean;
   DevBuf<double> invJ, inv_vol, fnormal, farea, Dt, Lt, w;
   DevBuf<int4> conn;
-  DevBuf<int> perm, fmap;
+  DevBuf<int> perm, fmap, job_off, jobs, pslot;
+  int ntiles = 0;
 
   ~Solver() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -111,6 +112,12 @@ struct Solver {
     farea.upload(m.face_area.data(), m.face_area.size());
     conn.upload(reinterpret_cast<const int4 *>(cn.conn.data()), (size_t)K * NF);
     perm.upload(cn.perm_tab.data(), cn.perm_tab.size());
+    const int epb = stage_epb(NP);
+    FaceJobs fj = build_face_jobs(m, re, cn, epb);
+    ntiles = (K + epb - 1) / epb;
+    job_off.upload(fj.job_off.data(), fj.job_off.size());
+    jobs.upload(fj.jobs.data(), fj.jobs.size());
+    pslot.upload(fj.pslot_tab.data(), fj.pslot_tab.size());
     std::vector<double> dt((size_t)dim * NP * NP), lt((size_t)NF * NFP * NP);
     for (int d = 0; d < dim; ++d)
       for (int i = 0; i < NP; ++i)
@@ -139,6 +146,7 @@ struct Solver {
     a.K = K;
     a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.conn = conn.p, a.fnormal = fnormal.p, a.farea = farea.p;
     a.perm = perm.p, a.Dt = Dt.p, a.Lt = Lt.p, a.fmap = fmap.p;
+    a.job_off = job_off.p, a.jobs = jobs.p, a.pslot = pslot.p, a.ntiles = ntiles;
     a.post_comp = -1;
     a.post_scale = 1.0;
     // GLM psi damping q[psi] *= exp(-alpha_d dt) once per step after the
@@ -299,7 +307,11 @@ int mxm_mesh_create_3d(const mxm_mesh3d_desc *d, int P, void **mesh) {
     if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
     check_bc(d->bc, 6);
     RefElem re = build_refelem(3, P);
-    *mesh = new HostMesh(build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re));
+    auto m = std::make_unique<HostMesh>(build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re));
+    // the kernel-side tables are validated here too, so host-only callers
+    // exercise the same checks a solver runs
+    build_face_jobs(*m, re, build_connectivity(*m, re), stage_epb(m->NP));
+    *mesh = m.release();
     return MXM_OK;
   });
 }
@@ -311,7 +323,9 @@ int mxm_mesh_create_2d(const mxm_mesh2d_desc *d, int P, void **mesh) {
     if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
     check_bc(d->bc, 4);
     RefElem re = build_refelem(2, P);
-    *mesh = new HostMesh(build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re));
+    auto m = std::make_unique<HostMesh>(build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re));
+    build_face_jobs(*m, re, build_connectivity(*m, re), stage_epb(m->NP));
+    *mesh = m.release();
     return MXM_OK;
   });
 }
@@ -492,7 +506,8 @@ int mxm_dg_memory_report(void *h, mxm_memory_report *o) {
     o->dg_operators = (long long)(s->Dt.bytes() + s->Lt.bytes() + s->w.bytes() + s->fmap.bytes());
     o->limiter = (long long)s->mean.bytes();
     o->mesh_connectivity = (long long)(s->invJ.bytes() + s->inv_vol.bytes() + s->conn.bytes() +
-                                       s->fnormal.bytes() + s->farea.bytes() + s->perm.bytes());
+                                       s->fnormal.bytes() + s->farea.bytes() + s->perm.bytes() +
+                                       s->job_off.bytes() + s->jobs.bytes() + s->pslot.bytes());
     o->halo_device = 0;
     o->halo_pinned = 0;
     return MXM_OK;

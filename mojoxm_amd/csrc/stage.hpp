@@ -46,9 +46,17 @@ struct StageArgs {
   const R *Dt;       // [ND][NP(j)][NP(i)]  transposed D_ref
   const R *Lt;       // [NF][NFP(r)][NP(i)] transposed Lift_ref
   const int *fmap;   // [NF][NFP] element node of each reference face slot
+  const int *job_off;  // [ntiles+1] face-flux jobs of each tile
+  const int *jobs;     // packed (el, lf, partner slot, partner lf, pslot)
+  const int *pslot;    // [npslot][NFP] partner reference slot per slot
+  int ntiles;        // tiles the job lists were built for
   int post_comp;     // component scaled after the combine (GLM psi), -1 none
   R post_scale;
 };
+
+// Elements per 256-thread block for NP nodes per element; the host builds
+// the face job lists with the same tiling.
+constexpr int stage_epb(int NP) { return 256 / NP; }
 
 template <int ND, int P>
 struct Shape {
@@ -56,7 +64,7 @@ struct Shape {
   static constexpr int NFP = ND == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
   static constexpr int NF = ND + 1;
   static constexpr int BLOCK = 256;
-  static constexpr int EPB = BLOCK / NP;
+  static constexpr int EPB = stage_epb(NP);
 };
 
 // Bijective map from the hardware block index (dispatched round-robin over
@@ -120,10 +128,16 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
         sF[((el * ND + k) * NP + j) * NC + c] = v;
       }
   }
-  // ---- 2b. face fluxes (each element evaluates its own faces) ------------
-  for (int t = tid; t < nE * NF * NFP; t += S::BLOCK) {
-    const int el = t / (NF * NFP), rem = t - el * NF * NFP;
-    const int lf = rem / NFP, r = rem - lf * NFP;
+  // ---- 2b. face fluxes: one evaluation per face inside the tile ---------
+  // A face whose two elements are both in this tile is evaluated once, by
+  // its side-0 element, which also stores the negated value into the
+  // partner's slot; faces to elements outside the tile are evaluated by
+  // each side (the reference evaluates every face from both sides).
+  const int j0 = A.job_off[tile], nitems = (A.job_off[tile + 1] - j0) * NFP;
+  for (int t = tid; t < nitems; t += S::BLOCK) {
+    const int jb = t / NFP, r = t - jb * NFP;
+    const int job = A.jobs[j0 + jb];
+    const int el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
     const int e = e0 + el;
     const int4 cn = A.conn[(size_t)e * NF + lf];
     const int side = cn.z & 1, bc = (cn.z >> 1) & 7, pid = cn.z >> 4;
@@ -162,6 +176,11 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
     const R s = side == 0 ? area : -area;
 #pragma unroll
     for (int c = 0; c < NC; ++c) sS[((el * NF + lf) * NFP + r) * NC + c] = s * Fs[c];
+    if (pin) {
+      const int plf = (job >> 16) & 3, rs = A.pslot[(job >> 18) * NFP + r];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sS[(((pin - 1) * NF + plf) * NFP + rs) * NC + c] = -(s * Fs[c]);
+    }
   }
   __syncthreads();
 
@@ -218,6 +237,7 @@ hipError_t launch_stage(const StageArgs<double> &a, const double *prm, hipStream
   const Phys ph = Phys::from(prm);
   const int nb = (a.K + S::EPB - 1) / S::EPB;
   if (nb <= 0) return hipSuccess;
+  if (nb != a.ntiles) return hipErrorInvalidValue;  // job lists built for another tiling
   hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, double>), dim3(nb), dim3(S::BLOCK), 0, s, a, ph);
   return hipGetLastError();
 }
