@@ -64,8 +64,9 @@ Connectivity build_connectivity(const HostMesh &m, const RefElem &re);
 // tiles of `epb` consecutive elements; within a tile, a face whose two
 // sides both lie in the tile is evaluated once (by its side-0 element),
 // which also writes the negated flux into the partner's slots.  For tile T
-// the jobs are jobs[job_off[T] .. job_off[T+1]), each packed as
-//   el | lf << 7 | (partner_local + 1 or 0) << 9 | partner_lf << 16 | pslot << 18
+// the jobs are jobs[4*job_off[T] .. 4*job_off[T+1]), four ints each:
+//   el | lf << 7 | (partner_local + 1 or 0) << 9 | partner_lf << 16 | pslot << 18,
+//   then the element's connectivity entry (partner, face id, flags)
 // and pslot_tab[pslot*NFP + r] is the partner's reference face slot that
 // meets slot r.
 struct FaceJobs {

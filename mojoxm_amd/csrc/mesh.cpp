@@ -524,8 +524,11 @@ FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivit
           packed |= ((pl + 1) << 9) | (plf << 16) | (id << 18);
         }
         J.jobs.push_back(packed);
+        J.jobs.push_back(cn[0]);
+        J.jobs.push_back(cn[1]);
+        J.jobs.push_back(cn[2]);
       }
-    J.job_off[T + 1] = (int)J.jobs.size();
+    J.job_off[T + 1] = (int)(J.jobs.size() / 4);
   }
   J.npslot = (int)pats.size();
   // Every (element, face, slot) of a tile must be written exactly once.
@@ -534,7 +537,7 @@ FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivit
     const int nE = std::min(epb, m.K - T * epb);
     std::fill(hits.begin(), hits.end(), 0);
     for (int j = J.job_off[T]; j < J.job_off[T + 1]; ++j) {
-      const int job = J.jobs[j], el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
+      const int job = J.jobs[(size_t)j * 4], el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
       if (el >= nE || (pin && pin - 1 >= nE)) throw std::runtime_error("jobs: element outside its tile");
       for (int r = 0; r < NFP; ++r) {
         ++hits[((size_t)el * NF + lf) * NFP + r];

@@ -5,7 +5,8 @@
 //   NC                        number of conserved components
 //   flux<ND>(q, F)            F[d*NC + c], world-frame flux, d < ND
 //   numerical_flux(ql,qr,n,f) two-state flux, n points from ql's element
-//   boundary_flux(q,bc,n,f)   flux on a boundary face, n outward
+//   ghost(q,bc,n,g)           ghost state of a boundary face (n outward);
+//                             boundary_flux = numerical_flux(q, ghost(q))
 //   source(q, s)              pointwise source (kSource)
 //   limit(q)                  post-stage positivity fix (kLimit)
 // Trait physics (Advection, Euler) run in 3D and, with nz = 0, on
@@ -46,24 +47,22 @@ struct Advection {
     R a = rabs(vn);
     f[0] = R(0.5) * ((vn + a) * ql[0] + (vn - a) * qr[0]);
   }
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
-    R a = rabs(vn);
-    R g = bc == BC_INFLOW ? inflow : R(0);
-    f[0] = R(0.5) * ((vn + a) * q[0] + (vn - a) * g);
-  }
+  // WALL / OUTFLOW: zero Dirichlet on the inflowing half; INFLOW: user value
+  // (src/advection.mojo:98-112)
+  MXM_HD void ghost(const R *, int bc, const R *, R *g) const { g[0] = bc == BC_INFLOW ? inflow : R(0); }
   MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
   MXM_HD void limit(R *) const {}
 };
 
 // ---------------------------------------------------------------- Euler --
-// src/euler.mojo:44-792.  FLUX: 0 Rusanov, 1 Roe, 2 HLLE, 3 HLLEC.
-template <class R, int FLUX>
+// src/euler.mojo:44-792.  FLUX: 0 Rusanov, 1 Roe, 2 HLLE, 3 HLLEC;
+// EFIX: Harten-Hyman entropy fix on the wave-based solvers.
+template <class R, int FLUX, int EFIX = 0>
 struct Euler {
   static constexpr int NC = 5;
   static constexpr bool kSource = true, kLimit = true;
+  static constexpr bool efix = EFIX != 0;
   R gamma, rmin_, pmin_;
-  int efix;
   R g[3];
   R inflow[5];
   static Euler from(const double *p) {
@@ -71,7 +70,6 @@ struct Euler {
     e.gamma = R(p[0]);
     e.rmin_ = R(p[1]);
     e.pmin_ = R(p[2]);
-    e.efix = p[4] != 0.0;
     for (int d = 0; d < 3; ++d) e.g[d] = R(p[5 + d]);
     for (int c = 0; c < 5; ++c) e.inflow[c] = R(p[8 + c]);
     return e;
@@ -126,18 +124,21 @@ struct Euler {
   // tangent / binormal, src/euler.mojo:94-118
   MXM_HD static Frame frame(const R *n) {
     Frame F;
-    R a[3] = {n[0], n[1], n[2]};
-    if (rabs(a[0]) >= R(0.9))
-      a[1] += R(1);
-    else
-      a[0] += R(1);
-    R pr = n[0] * a[0] + n[1] * a[1] + n[2] * a[2];
-    for (int d = 0; d < 3; ++d) a[d] -= n[d] * pr;
-    const R imag = R(1) / sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-    for (int d = 0; d < 3; ++d) {
-      F.n[d] = n[d];
-      F.t[d] = a[d] * imag;
-    }
+    // scalars, not an array: a select on the perturbed component would
+    // otherwise become a dynamically indexed private array (scratch)
+    const bool big = rabs(n[0]) >= R(0.9);
+    R ax = big ? n[0] : n[0] + R(1);
+    R ay = big ? n[1] + R(1) : n[1];
+    R az = n[2];
+    const R pr = n[0] * ax + n[1] * ay + n[2] * az;
+    ax -= n[0] * pr;
+    ay -= n[1] * pr;
+    az -= n[2] * pr;
+    const R imag = R(1) / sqrt(ax * ax + ay * ay + az * az);
+    for (int d = 0; d < 3; ++d) F.n[d] = n[d];
+    F.t[0] = ax * imag;
+    F.t[1] = ay * imag;
+    F.t[2] = az * imag;
     F.b[0] = n[1] * F.t[2] - n[2] * F.t[1];
     F.b[1] = n[2] * F.t[0] - n[0] * F.t[2];
     F.b[2] = n[0] * F.t[1] - n[1] * F.t[0];
@@ -279,7 +280,7 @@ struct Euler {
           }
         }
       }
-      if (efix)
+      if constexpr (efix)
         entropy_fixed(rl, rr, s, wave, amdq, apdq);
       else
         split(s, wave, amdq, apdq);
@@ -297,20 +298,16 @@ struct Euler {
     local_flux(rl, rr, rf);
     to_world(F, rf, f);
   }
-  // ghost states, src/euler.mojo:705-730
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R g[5];
-    if (bc == BC_WALL) {
-      R mn = q[1] * n[0] + q[2] * n[1] + q[3] * n[2];
-      g[0] = q[0];
-      g[4] = q[4];
-      for (int d = 0; d < 3; ++d) g[1 + d] = q[1 + d] - R(2) * mn * n[d];
-    } else if (bc == BC_INFLOW) {
-      for (int c = 0; c < 5; ++c) g[c] = inflow[c];
-    } else {
-      for (int c = 0; c < 5; ++c) g[c] = q[c];
-    }
-    numerical_flux(q, g, n, f);
+  // ghost states, src/euler.mojo:705-730: WALL reflects the normal
+  // momentum, INFLOW is the stored state, OUTFLOW copies
+  // (per-component selects: branches here become scratch-indexed stores)
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    const bool wall = bc == BC_WALL, in = bc == BC_INFLOW;
+    const R mn = q[1] * n[0] + q[2] * n[1] + q[3] * n[2];
+    g[0] = in ? inflow[0] : q[0];
+    g[4] = in ? inflow[4] : q[4];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[1 + d] = in ? inflow[1 + d] : (wall ? q[1 + d] - R(2) * mn * n[d] : q[1 + d]);
   }
   // gravity, src/euler.mojo:738-747
   MXM_HD void source(const R *q, R *s) const {
@@ -419,17 +416,13 @@ struct Euler2D {
       }
     }
   }
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R g[4];
-    if (bc == BC_WALL) {
-      R mn = q[1] * n[0] + q[2] * n[1];
-      g[0] = q[0], g[1] = q[1] - R(2) * mn * n[0], g[2] = q[2] - R(2) * mn * n[1], g[3] = q[3];
-    } else if (bc == BC_INFLOW) {
-      for (int c = 0; c < 4; ++c) g[c] = inflow[c];
-    } else {
-      for (int c = 0; c < 4; ++c) g[c] = q[c];
-    }
-    numerical_flux(q, g, n, f);
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    const bool wall = bc == BC_WALL, in = bc == BC_INFLOW;
+    const R mn = q[1] * n[0] + q[2] * n[1];
+    g[0] = in ? inflow[0] : q[0];
+    g[1] = in ? inflow[1] : (wall ? q[1] - R(2) * mn * n[0] : q[1]);
+    g[2] = in ? inflow[2] : (wall ? q[2] - R(2) * mn * n[1] : q[2]);
+    g[3] = in ? inflow[3] : q[3];
   }
   // gravity with the floored density, src/local_mesh_2d_gpu_euler.mojo:126-140
   MXM_HD void source(const R *q, R *s) const {
@@ -485,17 +478,12 @@ struct SW2D {
       }
     }
   }
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R g[3];
-    if (bc == BC_WALL) {
-      R mn = q[1] * n[0] + q[2] * n[1];
-      g[0] = q[0], g[1] = q[1] - R(2) * mn * n[0], g[2] = q[2] - R(2) * mn * n[1];
-    } else if (bc == BC_INFLOW) {
-      for (int c = 0; c < 3; ++c) g[c] = inflow[c];
-    } else {
-      for (int c = 0; c < 3; ++c) g[c] = q[c];
-    }
-    numerical_flux(q, g, n, f);
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    const bool wall = bc == BC_WALL, in = bc == BC_INFLOW;
+    const R mn = q[1] * n[0] + q[2] * n[1];
+    g[0] = in ? inflow[0] : q[0];
+    g[1] = in ? inflow[1] : (wall ? q[1] - R(2) * mn * n[0] : q[1]);
+    g[2] = in ? inflow[2] : (wall ? q[2] - R(2) * mn * n[1] : q[2]);
   }
   MXM_HD void source(const R *, R *s) const { s[0] = s[1] = s[2] = R(0); }
   MXM_HD void limit(R *) const {}
@@ -544,18 +532,17 @@ struct MHDGLM2D {
     for (int c = 0; c < 7; ++c)
       f[c] = R(0.5) * ((FL[c] + FR[c]) * n[0] + (FL[7 + c] + FR[7 + c]) * n[1]) - R(0.5) * al * (qr[c] - ql[c]);
   }
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R g[7];
-    if (bc == BC_WALL) {
-      R mn = q[1] * n[0] + q[2] * n[1], Bn = q[3] * n[0] + q[4] * n[1];
-      g[0] = q[0], g[1] = q[1] - R(2) * mn * n[0], g[2] = q[2] - R(2) * mn * n[1];
-      g[3] = q[3] - R(2) * Bn * n[0], g[4] = q[4] - R(2) * Bn * n[1], g[5] = q[5], g[6] = -q[6];
-    } else if (bc == BC_INFLOW) {
-      for (int c = 0; c < 7; ++c) g[c] = inflow[c];
-    } else {
-      for (int c = 0; c < 7; ++c) g[c] = q[c];
-    }
-    numerical_flux(q, g, n, f);
+  // WALL reflects m.n and B.n and flips psi (src/local_mesh_2d_gpu_mhd_glm.mojo:302-312)
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    const bool wall = bc == BC_WALL, in = bc == BC_INFLOW;
+    const R mn = q[1] * n[0] + q[2] * n[1], Bn = q[3] * n[0] + q[4] * n[1];
+    g[0] = in ? inflow[0] : q[0];
+    g[1] = in ? inflow[1] : (wall ? q[1] - R(2) * mn * n[0] : q[1]);
+    g[2] = in ? inflow[2] : (wall ? q[2] - R(2) * mn * n[1] : q[2]);
+    g[3] = in ? inflow[3] : (wall ? q[3] - R(2) * Bn * n[0] : q[3]);
+    g[4] = in ? inflow[4] : (wall ? q[4] - R(2) * Bn * n[1] : q[4]);
+    g[5] = in ? inflow[5] : q[5];
+    g[6] = in ? inflow[6] : (wall ? -q[6] : q[6]);
   }
   MXM_HD void source(const R *, R *s) const {
     for (int c = 0; c < 7; ++c) s[c] = R(0);
@@ -592,18 +579,16 @@ struct Maxwell2D {
     normal_flux(qr, c2, n[0], n[1], FR);
     for (int k = 0; k < 6; ++k) f[k] = R(0.5) * (FL[k] + FR[k]) - R(0.5) * c * (qr[k] - ql[k]);
   }
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R g[6];
-    if (bc == BC_WALL) {
-      R En = q[0] * n[0] + q[1] * n[1], Bn = q[3] * n[0] + q[4] * n[1];
-      g[0] = R(2) * En * n[0] - q[0], g[1] = R(2) * En * n[1] - q[1], g[2] = -q[2];
-      g[3] = q[3] - R(2) * Bn * n[0], g[4] = q[4] - R(2) * Bn * n[1], g[5] = q[5];
-    } else if (bc == BC_INFLOW) {
-      for (int k = 0; k < 6; ++k) g[k] = inflow[k];
-    } else {
-      for (int k = 0; k < 6; ++k) g[k] = q[k];
-    }
-    numerical_flux(q, g, n, f);
+  // PEC wall: tangential E and normal B flip (src/local_mesh_2d_gpu_maxwell.mojo:274-292)
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    const bool wall = bc == BC_WALL, in = bc == BC_INFLOW;
+    const R En = q[0] * n[0] + q[1] * n[1], Bn = q[3] * n[0] + q[4] * n[1];
+    g[0] = in ? inflow[0] : (wall ? R(2) * En * n[0] - q[0] : q[0]);
+    g[1] = in ? inflow[1] : (wall ? R(2) * En * n[1] - q[1] : q[1]);
+    g[2] = in ? inflow[2] : (wall ? -q[2] : q[2]);
+    g[3] = in ? inflow[3] : (wall ? q[3] - R(2) * Bn * n[0] : q[3]);
+    g[4] = in ? inflow[4] : (wall ? q[4] - R(2) * Bn * n[1] : q[4]);
+    g[5] = in ? inflow[5] : q[5];
   }
   // uniform J / M sources, src/local_mesh_2d_gpu_maxwell.mojo:133-142
   MXM_HD void source(const R *, R *s) const {
@@ -629,10 +614,9 @@ struct Advection2D {
     R vn = vx * n[0] + vy * n[1];
     f[0] = vn >= R(0) ? vn * ql[0] : vn * qr[0];
   }
-  MXM_HD void boundary_flux(const R *q, int bc, const R *n, R *f) const {
-    R vn = vx * n[0] + vy * n[1];
-    f[0] = vn >= R(0) ? vn * q[0] : (bc == BC_INFLOW ? vn * inflow : R(0));
-  }
+  // INFLOW: user value on the inflowing half; WALL / OUTFLOW: zero
+  // (src/local_mesh_2d_gpu_advection.mojo:285-298)
+  MXM_HD void ghost(const R *, int bc, const R *, R *g) const { g[0] = bc == BC_INFLOW ? inflow : R(0); }
   MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
   MXM_HD void limit(R *) const {}
 };

@@ -69,7 +69,8 @@ int phys_nc(int phys) {
 }
 
 int phys_variant(int phys, const double *prm) {
-  if (phys == MXM_PHYS_EULER) return (int)prm[3];
+  // Euler: flux kind + 4 * entropy fix (the fix only acts on the wave solvers)
+  if (phys == MXM_PHYS_EULER) return (int)prm[3] + (prm[3] != 0.0 && prm[4] != 0.0 ? 4 : 0);
   if (phys == MXM_PHYS_EULER2D) return prm[3] != 0.0 ? 1 : 0;
   if (phys == MXM_PHYS_SW2D) return prm[2] != 0.0 ? 1 : 0;
   return 0;
@@ -86,7 +87,8 @@ struct Solver {
   DevBuf<double> q, q1, q2, mean;
   DevBuf<double> invJ, inv_vol, fnormal, farea, Dt, Lt, w;
   DevBuf<int4> conn;
-  DevBuf<int> perm, fmap, job_off, jobs, pslot;
+  DevBuf<int> perm, fmap, job_off, pslot;
+  DevBuf<int4> jobs;
   int ntiles = 0;
 
   ~Solver() {
@@ -116,7 +118,7 @@ struct Solver {
     FaceJobs fj = build_face_jobs(m, re, cn, epb);
     ntiles = (K + epb - 1) / epb;
     job_off.upload(fj.job_off.data(), fj.job_off.size());
-    jobs.upload(fj.jobs.data(), fj.jobs.size());
+    jobs.upload(reinterpret_cast<const int4 *>(fj.jobs.data()), fj.jobs.size() / 4);
     pslot.upload(fj.pslot_tab.data(), fj.pslot_tab.size());
     std::vector<double> dt((size_t)dim * NP * NP), lt((size_t)NF * NFP * NP);
     for (int d = 0; d < dim; ++d)
@@ -144,7 +146,7 @@ struct Solver {
       a.cdt = (2.0 / 3.0) * dt;
     }
     a.K = K;
-    a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.conn = conn.p, a.fnormal = fnormal.p, a.farea = farea.p;
+    a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.fnormal = fnormal.p, a.farea = farea.p;
     a.perm = perm.p, a.Dt = Dt.p, a.Lt = Lt.p, a.fmap = fmap.p;
     a.job_off = job_off.p, a.jobs = jobs.p, a.pslot = pslot.p, a.ntiles = ntiles;
     a.post_comp = -1;

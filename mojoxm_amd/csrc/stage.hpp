@@ -11,14 +11,20 @@
 //
 // Work decomposition (256 threads = 4 wave64 per block, EPB elements per
 // block):
-//   1. the block's contiguous q_in tile, its geometry and (when small) the
-//      reference operators are staged in LDS with coalesced loads;
-//   2. per (element, node): the nodal flux, mapped to reference
-//      directions by invJ, into LDS;
-//      per (element, face, slot): the numerical or boundary flux, signed
-//      and scaled by the face area, into LDS -- the partner state comes
-//      from LDS when the neighbour is in the same tile, from global memory
-//      otherwise; side-0 is always the left state as in the reference;
+//   1. each thread starts the loads of its first face item (job record,
+//      face normal and area, partner node, and the partner's trace when the
+//      partner lies outside the tile); meanwhile the block stages its
+//      contiguous q_in tile, the tile's geometry and (when small) the
+//      reference operators in LDS with coalesced loads;
+//   2. per (element, node): the nodal flux, mapped to reference directions
+//      by invJ, into LDS;
+//      per face item: the numerical flux (boundary faces: against the
+//      physics' ghost state), signed and scaled by the face area, into LDS.
+//      A face whose two elements are both in the tile is evaluated once by
+//      its side-0 element, which also stores the negated value in the
+//      partner's slot; faces to elements outside the tile are evaluated by
+//      each side, as the reference evaluates every face from both sides;
+//      side 0 is always the left state;
 //   3. per (element, node): the D and Lift contractions over LDS, source,
 //      RK combine, limit_state, store.
 // Blocks are mapped to tiles so that each of the 8 XCDs walks a contiguous
@@ -37,20 +43,20 @@ struct StageArgs {
   R *q_out;
   R a, b, cdt;
   int K;
-  const R *invJ;     // [K][ND*ND] rows = reference directions
-  const R *inv_vol;  // [K] 1/(6V) or 1/(2A)
-  const int4 *conn;  // [K][NF] {partner, face id, side | bc<<1 | perm<<4, 0}
-  const R *fnormal;  // [Nf][ND]
-  const R *farea;    // [Nf]
-  const int *perm;   // [nperm][NFP] partner node per reference face slot
-  const R *Dt;       // [ND][NP(j)][NP(i)]  transposed D_ref
-  const R *Lt;       // [NF][NFP(r)][NP(i)] transposed Lift_ref
-  const int *fmap;   // [NF][NFP] element node of each reference face slot
+  const R *invJ;       // [K][ND*ND] rows = reference directions
+  const R *inv_vol;    // [K] 1/(6V) or 1/(2A)
+  const R *fnormal;    // [Nf][ND]
+  const R *farea;      // [Nf]
+  const int *perm;     // [nperm][NFP] partner node per reference face slot
+  const R *Dt;         // [ND][NP(j)][NP(i)]  transposed D_ref
+  const R *Lt;         // [NF][NFP(r)][NP(i)] transposed Lift_ref
+  const int *fmap;     // [NF][NFP] element node of each reference face slot
   const int *job_off;  // [ntiles+1] face-flux jobs of each tile
-  const int *jobs;     // packed (el, lf, partner slot, partner lf, pslot)
+  const int4 *jobs;    // {el | lf<<7 | pin<<9 | plf<<16 | pslot<<18, partner,
+                       //  face id, side | bc<<1 | perm<<4}
   const int *pslot;    // [npslot][NFP] partner reference slot per slot
-  int ntiles;        // tiles the job lists were built for
-  int post_comp;     // component scaled after the combine (GLM psi), -1 none
+  int ntiles;          // tiles the job lists were built for
+  int post_comp;       // component scaled after the combine (GLM psi), -1 none
   R post_scale;
 };
 
@@ -75,6 +81,38 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   return x < r ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
 }
 
+// One face item: slot r of a face job, with everything fetched from
+// global memory before the flux is evaluated.
+template <class R, int NC>
+struct FaceItem {
+  int packed, r, side, bc, pn, pl;
+  R n[3], area;
+  R qp[NC];  // partner trace when the partner is outside the tile
+};
+
+template <int ND, int NP, int NFP, int NC, class R>
+__device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int j0, int t, int e0, int nE,
+                                               FaceItem<R, NC> &it) {
+  const int jb = t / NFP;
+  it.r = t - jb * NFP;
+  const int4 jr = A.jobs[j0 + jb];
+  it.packed = jr.x;
+  it.side = jr.w & 1;
+  it.bc = (jr.w >> 1) & 7;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) it.n[d] = A.fnormal[(size_t)jr.z * ND + d];
+  if constexpr (ND == 2) it.n[2] = R(0);
+  it.area = A.farea[jr.z];
+  it.pn = A.perm[(jr.w >> 4) * NFP + it.r];
+  it.pl = jr.y - e0;
+  const bool need = !(it.side == 0 && it.bc != 0);
+  if (need && !(it.pl >= 0 && it.pl < nE)) {
+    const R *g = A.q_in + ((size_t)jr.y * NP + it.pn) * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) it.qp[c] = g[c];
+  }
+}
+
 template <class Phys, int ND, int P, class R>
 __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys) {
   using S = Shape<ND, P>;
@@ -82,9 +120,9 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   constexpr int NOPS = ND * NP * NP + NF * NP * NFP;
   constexpr bool kOpsLDS = NOPS * (int)sizeof(R) <= 24 * 1024;
   __shared__ R sQ[EPB * NP * NC];
-  __shared__ R sF[EPB * ND * NP * NC];      // [el][k][j][c]
-  __shared__ R sS[EPB * NF * NFP * NC];     // [el][f][r][c]
-  __shared__ R sJ[EPB * ND * ND + EPB];     // invJ then inv_vol
+  __shared__ R sF[EPB * ND * NP * NC];  // [el][k][j][c]
+  __shared__ R sS[EPB * NF * NFP * NC]; // [el][f][r][c]
+  __shared__ R sJ[EPB * ND * ND + EPB]; // invJ then inv_vol
   __shared__ R sOp[kOpsLDS ? NOPS : 1];
   __shared__ int sMap[NF * NFP];
 
@@ -93,8 +131,12 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   const int e0 = tile * EPB;
   const int nE = min(EPB, A.K - e0);
   if (nE <= 0) return;
+  const int j0 = A.job_off[tile];
+  const int nitems = (A.job_off[tile + 1] - j0) * NFP;
 
-  // ---- 1. stage the tile ------------------------------------------------
+  // ---- 1. first face item of this thread, then the tile ------------------
+  FaceItem<R, NC> it;
+  if (tid < nitems) load_face_item<ND, NP, NFP, NC>(A, j0, tid, e0, nE, it);
   {
     const R *src = A.q_in + (size_t)e0 * NP * NC;
     const int n = nE * NP * NC;
@@ -128,56 +170,38 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
         sF[((el * ND + k) * NP + j) * NC + c] = v;
       }
   }
-  // ---- 2b. face fluxes: one evaluation per face inside the tile ---------
-  // A face whose two elements are both in this tile is evaluated once, by
-  // its side-0 element, which also stores the negated value into the
-  // partner's slot; faces to elements outside the tile are evaluated by
-  // each side (the reference evaluates every face from both sides).
-  const int j0 = A.job_off[tile], nitems = (A.job_off[tile + 1] - j0) * NFP;
+
+  // ---- 2b. face fluxes ----------------------------------------------------
   for (int t = tid; t < nitems; t += S::BLOCK) {
-    const int jb = t / NFP, r = t - jb * NFP;
-    const int job = A.jobs[j0 + jb];
-    const int el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
-    const int e = e0 + el;
-    const int4 cn = A.conn[(size_t)e * NF + lf];
-    const int side = cn.z & 1, bc = (cn.z >> 1) & 7, pid = cn.z >> 4;
-    const R *qme = &sQ[(el * NP + sMap[lf * NFP + r]) * NC];
-    R n[3];
+    if (t != tid) load_face_item<ND, NP, NFP, NC>(A, j0, t, e0, nE, it);
+    const int el = it.packed & 127, lf = (it.packed >> 7) & 3, pin = (it.packed >> 9) & 127;
+    R qm[NC], qo[NC], ql[NC], qr[NC];
+    const int mine = (el * NP + sMap[lf * NFP + it.r]) * NC;
 #pragma unroll
-    for (int d = 0; d < ND; ++d) n[d] = A.fnormal[(size_t)cn.y * ND + d];
-    if constexpr (ND == 2) n[2] = R(0);
-    const R area = A.farea[cn.y];
-    R qp[NC];
-    const bool need_partner = !(side == 0 && bc != 0);
-    if (need_partner) {
-      const int pe = cn.x, pn = A.perm[pid * NFP + r];
-      const int pl = pe - e0;
-      if (pl >= 0 && pl < nE) {
+    for (int c = 0; c < NC; ++c) qm[c] = sQ[mine + c];
+    if (it.pl >= 0 && it.pl < nE) {
+      const int other = (it.pl * NP + it.pn) * NC;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) qp[c] = sQ[(pl * NP + pn) * NC + c];
-      } else {
-        const R *g = A.q_in + ((size_t)pe * NP + pn) * NC;
-#pragma unroll
-        for (int c = 0; c < NC; ++c) qp[c] = g[c];
-      }
-    }
-    R Fs[NC];
-    if (side == 0) {
-      if (bc)
-        phys.boundary_flux(qme, bc, n, Fs);
-      else
-        phys.numerical_flux(qme, qp, n, Fs);
+      for (int c = 0; c < NC; ++c) qo[c] = sQ[other + c];
     } else {
-      if (bc)
-        phys.boundary_flux(qp, bc, n, Fs);
-      else
-        phys.numerical_flux(qp, qme, n, Fs);
-    }
-    const R s = side == 0 ? area : -area;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sS[((el * NF + lf) * NFP + r) * NC + c] = s * Fs[c];
+      for (int c = 0; c < NC; ++c) qo[c] = it.qp[c];
+    }
+    // left = side-0 trace; right = side-1 trace or the ghost of the left
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ql[c] = it.side == 0 ? qm[c] : qo[c];
+    if (it.bc)
+      phys.ghost(ql, it.bc, it.n, qr);
+    else
+#pragma unroll
+      for (int c = 0; c < NC; ++c) qr[c] = it.side == 0 ? qo[c] : qm[c];
+    R Fs[NC];
+    phys.numerical_flux(ql, qr, it.n, Fs);
+    const R s = it.side == 0 ? it.area : -it.area;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sS[((el * NF + lf) * NFP + it.r) * NC + c] = s * Fs[c];
     if (pin) {
-      const int plf = (job >> 16) & 3, rs = A.pslot[(job >> 18) * NFP + r];
+      const int plf = (it.packed >> 16) & 3, rs = A.pslot[(it.packed >> 18) * NFP + it.r];
 #pragma unroll
       for (int c = 0; c < NC; ++c) sS[(((pin - 1) * NF + plf) * NFP + rs) * NC + c] = -(s * Fs[c]);
     }

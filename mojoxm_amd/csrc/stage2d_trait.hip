@@ -14,6 +14,9 @@ StageLaunch pick(int phys, int variant) {
       case 1: return &launch_stage<Euler<double, 1>, 2, P>;
       case 2: return &launch_stage<Euler<double, 2>, 2, P>;
       case 3: return &launch_stage<Euler<double, 3>, 2, P>;
+      case 5: return &launch_stage<Euler<double, 1, 1>, 2, P>;
+      case 6: return &launch_stage<Euler<double, 2, 1>, 2, P>;
+      case 7: return &launch_stage<Euler<double, 3, 1>, 2, P>;
     }
   }
   return nullptr;

@@ -1,5 +1,6 @@
 // stage3d.hip -- stage-kernel instantiations for the Kuhn-tetrahedron mesh
-// (trait physics: Advection, Euler with each Riemann solver), P = 1..5.
+// (trait physics: Advection, Euler with each Riemann solver, with and
+// without the entropy fix), P = 1..5.
 #include "stage.hpp"
 
 namespace mxm {
@@ -13,6 +14,9 @@ StageLaunch pick(int phys, int variant) {
       case 1: return &launch_stage<Euler<double, 1>, 3, P>;
       case 2: return &launch_stage<Euler<double, 2>, 3, P>;
       case 3: return &launch_stage<Euler<double, 3>, 3, P>;
+      case 5: return &launch_stage<Euler<double, 1, 1>, 3, P>;
+      case 6: return &launch_stage<Euler<double, 2, 1>, 3, P>;
+      case 7: return &launch_stage<Euler<double, 3, 1>, 3, P>;
     }
   }
   return nullptr;
