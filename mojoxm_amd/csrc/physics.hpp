@@ -310,6 +310,7 @@ struct Euler {
     for (int d = 0; d < 3; ++d) g[1 + d] = in ? inflow[1 + d] : (wall ? q[1 + d] - R(2) * mn * n[d] : q[1 + d]);
   }
   // gravity, src/euler.mojo:738-747
+  MXM_HD bool source_active() const { return g[0] != R(0) || g[1] != R(0) || g[2] != R(0); }
   MXM_HD void source(const R *q, R *s) const {
     s[0] = R(0);
     s[1] = q[0] * g[0];
@@ -425,6 +426,7 @@ struct Euler2D {
     g[3] = in ? inflow[3] : q[3];
   }
   // gravity with the floored density, src/local_mesh_2d_gpu_euler.mojo:126-140
+  MXM_HD bool source_active() const { return gx != R(0) || gy != R(0); }
   MXM_HD void source(const R *q, R *s) const {
     R r = q[0] < rmin_ ? rmin_ : q[0];
     s[0] = R(0), s[1] = r * gx, s[2] = r * gy, s[3] = q[1] * gx + q[2] * gy;
@@ -591,6 +593,7 @@ struct Maxwell2D {
     g[5] = in ? inflow[5] : q[5];
   }
   // uniform J / M sources, src/local_mesh_2d_gpu_maxwell.mojo:133-142
+  MXM_HD bool source_active() const { return true; }
   MXM_HD void source(const R *, R *s) const {
     R c2 = c * c;
     for (int d = 0; d < 3; ++d) s[d] = -(c2 * J[d]), s[3 + d] = -M[d];

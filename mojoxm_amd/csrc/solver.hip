@@ -120,13 +120,15 @@ struct Solver {
     job_off.upload(fj.job_off.data(), fj.job_off.size());
     jobs.upload(reinterpret_cast<const int4 *>(fj.jobs.data()), fj.jobs.size() / 4);
     pslot.upload(fj.pslot_tab.data(), fj.pslot_tab.size());
-    std::vector<double> dt((size_t)dim * NP * NP), lt((size_t)NF * NFP * NP);
+    // transposed operators, node columns zero-padded as the kernel expects
+    const int npad = stage_node_pad(NP, NC);
+    std::vector<double> dt((size_t)dim * NP * npad, 0.0), lt((size_t)NF * NFP * npad, 0.0);
     for (int d = 0; d < dim; ++d)
       for (int i = 0; i < NP; ++i)
-        for (int j = 0; j < NP; ++j) dt[((size_t)d * NP + j) * NP + i] = re.D[((size_t)d * NP + i) * NP + j];
+        for (int j = 0; j < NP; ++j) dt[((size_t)d * NP + j) * npad + i] = re.D[((size_t)d * NP + i) * NP + j];
     for (int f = 0; f < NF; ++f)
       for (int i = 0; i < NP; ++i)
-        for (int r = 0; r < NFP; ++r) lt[((size_t)f * NFP + r) * NP + i] = re.Lift[((size_t)f * NP + i) * NFP + r];
+        for (int r = 0; r < NFP; ++r) lt[((size_t)f * NFP + r) * npad + i] = re.Lift[((size_t)f * NP + i) * NFP + r];
     Dt.upload(dt.data(), dt.size());
     Lt.upload(lt.data(), lt.size());
     w.upload(re.w.data(), re.w.size());
@@ -137,13 +139,14 @@ struct Solver {
   // Shu-Osher plan of src/ssprk3.mojo:41-78 (FP64 coefficients).
   void stage(int s, double dt) {
     StageArgs<double> a{};
+    // (q_in, q_a, q_b, q_out): (q, q, q, q1), (q1, q, q1, q2), (q2, q, q2, q);
+    // q_b is always the stage input, which the kernel reads from its tile
     if (s == 0) {
-      a.q_in = q.p, a.q_a = q.p, a.q_b = q.p, a.q_out = q1.p, a.a = 1.0, a.b = 0.0, a.cdt = 1.0 * dt;
+      a.q_in = q.p, a.q_a = q.p, a.q_out = q1.p, a.a = 1.0, a.b = 0.0, a.cdt = 1.0 * dt;
     } else if (s == 1) {
-      a.q_in = q1.p, a.q_a = q.p, a.q_b = q1.p, a.q_out = q2.p, a.a = 0.75, a.b = 0.25, a.cdt = 0.25 * dt;
+      a.q_in = q1.p, a.q_a = q.p, a.q_out = q2.p, a.a = 0.75, a.b = 0.25, a.cdt = 0.25 * dt;
     } else {
-      a.q_in = q2.p, a.q_a = q.p, a.q_b = q2.p, a.q_out = q.p, a.a = 1.0 / 3.0, a.b = 2.0 / 3.0,
-      a.cdt = (2.0 / 3.0) * dt;
+      a.q_in = q2.p, a.q_a = q.p, a.q_out = q.p, a.a = 1.0 / 3.0, a.b = 2.0 / 3.0, a.cdt = (2.0 / 3.0) * dt;
     }
     a.K = K;
     a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.fnormal = fnormal.p, a.farea = farea.p;

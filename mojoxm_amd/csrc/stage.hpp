@@ -37,9 +37,11 @@
 
 namespace mxm {
 
+// Shu-Osher stage q_out = a q_a + b q_in + c dt L(q_in): in every SSPRK3
+// stage of src/ssprk3.mojo:41-78 the b-term buffer is the stage input.
 template <class R>
 struct StageArgs {
-  const R *q_in, *q_a, *q_b;
+  const R *q_in, *q_a;
   R *q_out;
   R a, b, cdt;
   int K;
@@ -48,8 +50,8 @@ struct StageArgs {
   const R *fnormal;    // [Nf][ND]
   const R *farea;      // [Nf]
   const int *perm;     // [nperm][NFP] partner node per reference face slot
-  const R *Dt;         // [ND][NP(j)][NP(i)]  transposed D_ref
-  const R *Lt;         // [NF][NFP(r)][NP(i)] transposed Lift_ref
+  const R *Dt;         // [ND*NP (k,j)][NPAD (i)] transposed D_ref, zero padded
+  const R *Lt;         // [NF*NFP (f,r)][NPAD (i)] transposed Lift_ref, zero padded
   const int *fmap;     // [NF][NFP] element node of each reference face slot
   const int *job_off;  // [ntiles+1] face-flux jobs of each tile
   const int4 *jobs;    // {el | lf<<7 | pin<<9 | plf<<16 | pslot<<18, partner,
@@ -63,6 +65,18 @@ struct StageArgs {
 // Elements per 256-thread block for NP nodes per element; the host builds
 // the face job lists with the same tiling.
 constexpr int stage_epb(int NP) { return 256 / NP; }
+
+// Contraction layout: the tile's (element, component) columns span
+// ceil(EPB*NC/64) waves; the block's 4 waves form `groups` groups, each
+// producing `IH` consecutive output nodes, so the operator entries a wave
+// reads are wave-uniform.  The transposed operators are stored with
+// groups*IH (>= NP) zero-padded node columns.
+constexpr int stage_groups(int NP, int NC) {
+  return (stage_epb(NP) * NC + 63) / 64 >= 4 ? 1 : 4 / ((stage_epb(NP) * NC + 63) / 64);
+}
+constexpr int stage_node_pad(int NP, int NC) {
+  return stage_groups(NP, NC) * ((NP + stage_groups(NP, NC) - 1) / stage_groups(NP, NC));
+}
 
 template <int ND, int P>
 struct Shape {
@@ -117,13 +131,18 @@ template <class Phys, int ND, int P, class R>
 __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys) {
   using S = Shape<ND, P>;
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
-  constexpr int NOPS = ND * NP * NP + NF * NP * NFP;
-  constexpr bool kOpsLDS = NOPS * (int)sizeof(R) <= 24 * 1024;
+  // Contraction layout: the (element, component) columns of the tile are
+  // spread over one or more waves; the block's waves split into GROUPS
+  // groups that each produce a contiguous range of IH output nodes, so the
+  // operator entries a wave needs are wave-uniform (scalar loads).
+  constexpr int GROUPS = stage_groups(NP, NC);
+  constexpr int GSIZE = S::BLOCK / GROUPS;
+  constexpr int NPAD = stage_node_pad(NP, NC);
+  constexpr int IH = NPAD / GROUPS;
   __shared__ R sQ[EPB * NP * NC];
-  __shared__ R sF[EPB * ND * NP * NC];  // [el][k][j][c]
-  __shared__ R sS[EPB * NF * NFP * NC]; // [el][f][r][c]
+  __shared__ R sF[ND * NP * EPB * NC];  // [k][j][el][c]; later the rhs [el][i][c]
+  __shared__ R sS[NF * NFP * EPB * NC]; // [f*NFP + r][el][c]
   __shared__ R sJ[EPB * ND * ND + EPB]; // invJ then inv_vol
-  __shared__ R sOp[kOpsLDS ? NOPS : 1];
   __shared__ int sMap[NF * NFP];
 
   const int tid = threadIdx.x;
@@ -137,6 +156,13 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   // ---- 1. first face item of this thread, then the tile ------------------
   FaceItem<R, NC> it;
   if (tid < nitems) load_face_item<ND, NP, NFP, NC>(A, j0, tid, e0, nE, it);
+  // q_a of this thread's output node (EPB*NP <= 256: one node per thread)
+  R qa[NC];
+  if (tid < nE * NP) {
+    const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qa[c] = g[c];
+  }
   {
     const R *src = A.q_in + (size_t)e0 * NP * NC;
     const int n = nE * NP * NC;
@@ -145,14 +171,8 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
     for (int t = tid; t < nE * ND * ND; t += S::BLOCK) sJ[t] = gJ[t];
     for (int t = tid; t < nE; t += S::BLOCK) sJ[EPB * ND * ND + t] = A.inv_vol[e0 + t];
     for (int t = tid; t < NF * NFP; t += S::BLOCK) sMap[t] = A.fmap[t];
-    if constexpr (kOpsLDS) {
-      for (int t = tid; t < ND * NP * NP; t += S::BLOCK) sOp[t] = A.Dt[t];
-      for (int t = tid; t < NF * NP * NFP; t += S::BLOCK) sOp[ND * NP * NP + t] = A.Lt[t];
-    }
   }
   __syncthreads();
-  const R *Dop = kOpsLDS ? sOp : A.Dt;
-  const R *Lop = kOpsLDS ? sOp + ND * NP * NP : A.Lt;
 
   // ---- 2a. contravariant nodal fluxes ------------------------------------
   for (int t = tid; t < nE * NP; t += S::BLOCK) {
@@ -167,7 +187,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
         R v = R(0);
 #pragma unroll
         for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * F[d * NC + c];
-        sF[((el * ND + k) * NP + j) * NC + c] = v;
+        sF[((k * NP + j) * EPB + el) * NC + c] = v;
       }
   }
 
@@ -199,53 +219,72 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
     phys.numerical_flux(ql, qr, it.n, Fs);
     const R s = it.side == 0 ? it.area : -it.area;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sS[((el * NF + lf) * NFP + it.r) * NC + c] = s * Fs[c];
+    for (int c = 0; c < NC; ++c) sS[((lf * NFP + it.r) * EPB + el) * NC + c] = s * Fs[c];
     if (pin) {
       const int plf = (it.packed >> 16) & 3, rs = A.pslot[(it.packed >> 18) * NFP + it.r];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sS[(((pin - 1) * NF + plf) * NFP + rs) * NC + c] = -(s * Fs[c]);
+      for (int c = 0; c < NC; ++c) sS[((plf * NFP + rs) * EPB + (pin - 1)) * NC + c] = -(s * Fs[c]);
     }
   }
   __syncthreads();
 
-  // ---- 3. contractions, source, RK combine, limit, store ----------------
-  for (int t = tid; t < nE * NP; t += S::BLOCK) {
-    const int el = t / NP, i = t - el * NP;
-    R vol[NC], fac[NC];
+  // ---- 3a. contractions per (element, component) column ------------------
+  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr,
+  // written into sQ (no longer read after the face phase) as [el][i][c].
+  {
+    const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
+    const int i0 = grp * IH;
+    for (int p = tid - grp * GSIZE; p < nE * NC; p += GSIZE) {
+      const int el = p / NC, c = p - el * NC;
+      R acc[IH];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) vol[c] = fac[c] = R(0);
-    for (int k = 0; k < ND; ++k) {
-      const R *Fk = &sF[(el * ND + k) * NP * NC];
-      const R *Dk = Dop + k * NP * NP;
+      for (int ii = 0; ii < IH; ++ii) acc[ii] = R(0);
 #pragma unroll 4
-      for (int j = 0; j < NP; ++j) {
-        const R dv = Dk[j * NP + i];
+      for (int fr = 0; fr < NF * NFP; ++fr) {
+        const R sv = sS[fr * EPB * NC + p];
+        const R *L = A.Lt + fr * NPAD + i0;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) vol[c] += dv * Fk[j * NC + c];
+        for (int ii = 0; ii < IH; ++ii) acc[ii] += L[ii] * sv;
+      }
+      const R niv = -sJ[EPB * ND * ND + el];
+#pragma unroll
+      for (int ii = 0; ii < IH; ++ii) acc[ii] *= niv;
+#pragma unroll 4
+      for (int kj = 0; kj < ND * NP; ++kj) {
+        const R fv = sF[kj * EPB * NC + p];
+        const R *D = A.Dt + kj * NPAD + i0;
+#pragma unroll
+        for (int ii = 0; ii < IH; ++ii) acc[ii] += D[ii] * fv;
+      }
+#pragma unroll
+      for (int ii = 0; ii < IH; ++ii)
+        if (i0 + ii < NP) {
+          // the slot holds q_in of this (node, component): fold in b q_in
+          R &slot = sQ[(el * NP + i0 + ii) * NC + c];
+          slot = A.b * slot + A.cdt * acc[ii];
+        }
+    }
+  }
+  __syncthreads();
+
+  // ---- 3b. source, RK combine, limit, store per (element, node) ---------
+  if (tid < nE * NP) {
+    const int t = tid;
+    const size_t g = (size_t)e0 * NP * NC + (size_t)t * NC;
+    R src[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) src[c] = R(0);
+    if constexpr (Phys::kSource) {
+      if (phys.source_active()) {
+        R qi[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) qi[c] = A.q_in[g + c];
+        phys.source(qi, src);
       }
     }
-    const R *Se = &sS[el * NF * NFP * NC];
-#pragma unroll 4
-    for (int fr = 0; fr < NF * NFP; ++fr) {
-      const R lv = Lop[fr * NP + i];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) fac[c] += lv * Se[fr * NC + c];
-    }
-    const R iv = sJ[EPB * ND * ND + el];
-    const R *qi = &sQ[(el * NP + i) * NC];
-    R src[NC];
-    if constexpr (Phys::kSource)
-      phys.source(qi, src);
-    else
-#pragma unroll
-      for (int c = 0; c < NC; ++c) src[c] = R(0);
-    const size_t g = ((size_t)(e0 + el) * NP + i) * NC;
     R out[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const R rhs = vol[c] - iv * fac[c] + src[c];
-      out[c] = A.a * A.q_a[g + c] + A.b * A.q_b[g + c] + A.cdt * rhs;
-    }
+    for (int c = 0; c < NC; ++c) out[c] = A.a * qa[c] + sQ[t * NC + c] + A.cdt * src[c];
     if constexpr (Phys::kLimit) phys.limit(out);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
