@@ -269,7 +269,8 @@ int mxm_get_constant(int which, int *value) {
   if (which == 1) {
     int n = 0;
     for (int P = 1; P <= 5; ++P) {
-      for (int v = 0; v < 4; ++v) n += (find_stage_3d(1, v, P) != nullptr) + (find_stage_2d_trait(1, v, P) != nullptr);
+      // Euler variants: flux kind + 4 * entropy fix
+      for (int v = 0; v < 8; ++v) n += (find_stage_3d(1, v, P) != nullptr) + (find_stage_2d_trait(1, v, P) != nullptr);
       n += (find_stage_3d(0, 0, P) != nullptr) + (find_stage_2d_trait(0, 0, P) != nullptr);
       for (int ph = 2; ph <= 6; ++ph)
         for (int v = 0; v < 2; ++v)
