@@ -22,13 +22,17 @@
  *               damping src/local_mesh_2d_gpu_mhd_glm.mojo:464-484
  *
  * Pinned by tests/test_oracle_stage.py: constant-state preservation for
- * every physics (test/euler_2d_gpu_test.mojo:5-18,
- * test/euler_3d_test.mojo:1-16), exact periodic conservation of
- * sum_i w_i q_i / inv_vol, the exact derivative of a linear field, the
- * SSPRK3 amplification polynomial 1+z+z^2/2+z^3/6, limiter passthrough /
- * mean conservation / spike clamp (test/limiter_2d_gpu_test_p3.mojo:14-19,
- * test/limiter_3d_test.mojo:8-24) and the analytic Gaussian return of the
- * 3D advection case.
+ * every physics and P (test/euler_2d_gpu_test.mojo:5-18,
+ * test/euler_3d_test.mojo:1-16 and the SW / Maxwell / GLM siblings),
+ * exact periodic conservation of sum_i w_i q_i, the stage combine and the
+ * SSPRK3 step as the cubic Taylor polynomial of the (linear) advection
+ * operator, convergence of the DG derivative at rate >= P, closed-form
+ * gravity / Maxwell J,M sources and GLM psi damping, the Gaussian return
+ * (3D advection), the translated vortex, the PEC cavity TM(1,1) mode, the
+ * Sod star-region values of test/sod_exact_riemann_test.mojo:16-19 with the
+ * limiter on, and the limiter checks of test/limiter_2d_gpu_test_p3.mojo
+ * and test/limiter_3d_test.mojo (passthrough, spike clamp with the far
+ * field untouched, cell-mean conservation) plus its scaling form.
  */
 #include <math.h>
 #include <stdlib.h>
