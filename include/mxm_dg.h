@@ -177,10 +177,10 @@ int mxm_dg_bench_step_loop(void *solver, double dt, int warmup, int measure,
 int mxm_dg_dof_count(void *solver, long long *out);
 int mxm_dg_state_bytes_per_step(void *solver, long long *out);
 int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
-/* Average duration (microseconds) of the stage kernel over nsteps SSPRK3
- * steps, timed with HIP events on the solver's own stream; advances the
- * state.  No reference counterpart: it is the measurement hook that the
- * roofline figures are computed from. */
+/* Average duration (microseconds) of the stage kernel over 3*nsteps
+ * back-to-back launches (limiter off), from two HIP events on the solver's
+ * own stream around them; advances the state.  No reference counterpart:
+ * it is the measurement hook that the roofline figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
 int mxm_dg_destroy(void *solver);
 

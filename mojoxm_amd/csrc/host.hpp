@@ -60,19 +60,41 @@ struct Connectivity {
 // a face slot is its own reference-face node) and compresses them.
 Connectivity build_connectivity(const HostMesh &m, const RefElem &re);
 
-// Face-flux work lists of the stage kernel.  Elements are processed in
-// tiles of `epb` consecutive elements; within a tile, a face whose two
-// sides both lie in the tile is evaluated once (by its side-0 element),
-// which also writes the negated flux into the partner's slots.  For tile T
-// the jobs are jobs[4*job_off[T] .. 4*job_off[T+1]), four ints each:
-//   el | lf << 7 | (partner_local + 1 or 0) << 9 | partner_lf << 16 | pslot << 18,
-//   then the element's connectivity entry (partner, face id, flags)
-// and pslot_tab[pslot*NFP + r] is the partner's reference face slot that
-// meets slot r.
+// Element tiles of the stage kernel and the device element order.  A tile
+// is a block of lattice cells (squares of 2 triangles, cubes of 6 tets),
+// at most `epb` elements, whose shape minimises the face-flux work per
+// element (faces with both sides in the tile are evaluated once).  Device
+// element slot T*epb + el holds element ref_of[T*epb + el] (-1: padding),
+// so every tile is contiguous in device memory; dev_of is the inverse.
+// When one cell has more elements than a tile (3D, P = 5), tiles are runs
+// of consecutive elements and block = {0, 0, 0}.
+struct Tiling {
+  int epb = 0, ntiles = 0;
+  int block[3] = {1, 1, 1};  // cells per tile along x, y, z
+  std::vector<int32_t> dev_of, ref_of, count;
+  int slots() const { return ntiles * epb; }
+};
+Tiling build_tiling(const HostMesh &m, int epb, int NFP);
+
+// Face-flux work lists of the stage kernel.  Within a tile, a face whose
+// two sides both lie in the tile is evaluated once (by its side-0
+// element), which also writes the negated flux into the partner's slots.
+// Every tile owns `maxj` job slots (padding: packed = -1), so the records a
+// thread needs are addressed from the tile index alone:
+//   jobs[T*maxj + j]  el | lf << 7 | (partner_local + 1 or 0) << 9 |
+//                     partner_lf << 16 | pslot << 18, the partner's device
+//                     slot, the face id, the connectivity flags
+//   geo[(T*maxj + j)*4 + 0..3]  face normal (x, y, z; 0 beyond ND), area
+//   ipr[(T*maxj + j)*NFP + r]   partner node meeting slot r | (partner's
+//                     reference slot for the paired write) << 8
+// pslot_tab[pslot*NFP + r] is the partner's reference slot meeting slot r.
 struct FaceJobs {
-  std::vector<int32_t> job_off, jobs, pslot_tab;
+  int maxj = 0;
+  std::vector<int32_t> jobs, pslot_tab;
+  std::vector<double> geo;
+  std::vector<uint16_t> ipr;
   int npslot = 0;
 };
-FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivity &c, int epb);
+FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivity &c, const Tiling &t);
 
 }  // namespace mxm

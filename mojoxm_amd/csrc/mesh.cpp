@@ -476,22 +476,95 @@ Connectivity build_connectivity(const HostMesh &m, const RefElem &re) {
   return c;
 }
 
-FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivity &c, int epb) {
-  const int NF = m.NF, NFP = m.NFP;
-  if (epb < 1 || epb > 126) throw std::runtime_error("jobs: tile size out of range");
+Tiling build_tiling(const HostMesh &m, int epb, int NFP) {
+  if (epb < 1 || epb > 126) throw std::runtime_error("tiling: tile size out of range");
+  const int dim = m.dim, epc = dim == 3 ? 6 : 2;  // elements per lattice cell
+  const int nx = m.n[0], ny = m.n[1], nz = dim == 3 ? m.n[2] : 1;
+  if ((long long)nx * ny * nz * epc != m.K) throw std::runtime_error("tiling: element count is not a lattice");
+  // Faces evaluated for a bx x by x bz block on a periodic lattice: pairs
+  // inside the block once, faces to other blocks once per side.
+  const int inner = dim == 3 ? 6 : 1, per_iface = dim == 3 ? 2 : 1;
+  Tiling t;
+  t.epb = epb;
+  double best = 1e30;
+  int best_faces = 0, best_cells = 0;
+  for (int bz = 1; bz <= nz; ++bz)
+    for (int by = 1; by <= ny; ++by)
+      for (int bx = 1; bx <= nx; ++bx) {
+        const int cells = bx * by * bz;
+        if (cells * epc > epb) continue;
+        const int ifaces = (bx - 1) * by * bz + bx * (by - 1) * bz + (dim == 3 ? bx * by * (bz - 1) : 0);
+        const int sfaces = 2 * (by * bz + bx * bz + (dim == 3 ? bx * by : 0));
+        const int faces = cells * inner + ifaces * per_iface + sfaces * per_iface;
+        // wave-executions of the face phase plus about one for the per-node
+        // phases of the block, per element
+        const int waves = (faces * NFP + 63) / 64;
+        const double cost = (double)(waves + 1) / (cells * epc);
+        if (cost < best - 1e-12 || (cost < best + 1e-12 && (faces * best_cells < best_faces * cells ||
+                                                           (faces * best_cells == best_faces * cells && cells > best_cells)))) {
+          best = cost, best_faces = faces, best_cells = cells;
+          t.block[0] = bx, t.block[1] = by, t.block[2] = bz;
+        }
+      }
+  t.dev_of.assign(m.K, -1);
+  if (best_cells == 0) {
+    // a cell has more elements than a tile holds: consecutive elements
+    t.block[0] = t.block[1] = t.block[2] = 0;
+    t.ntiles = (m.K + epb - 1) / epb;
+    t.ref_of.assign((size_t)t.ntiles * epb, -1);
+    t.count.assign(t.ntiles, 0);
+    for (int e = 0; e < m.K; ++e) {
+      t.ref_of[e] = e;
+      t.dev_of[e] = e;
+      ++t.count[e / epb];
+    }
+    return t;
+  }
+  const int bx = t.block[0], by = t.block[1], bz = t.block[2];
+  const int tx = (nx + bx - 1) / bx, ty = (ny + by - 1) / by, tz = (nz + bz - 1) / bz;
+  t.ntiles = tx * ty * tz;
+  t.ref_of.assign((size_t)t.ntiles * epb, -1);
+  t.count.assign(t.ntiles, 0);
+  // tiles x-fastest; cells inside a tile z, y, x order; then the cell's elements
+  for (int TZ = 0; TZ < tz; ++TZ)
+    for (int TY = 0; TY < ty; ++TY)
+      for (int TX = 0; TX < tx; ++TX) {
+        const int T = (TZ * ty + TY) * tx + TX;
+        int el = 0;
+        for (int k = TZ * bz; k < std::min(nz, (TZ + 1) * bz); ++k)
+          for (int j = TY * by; j < std::min(ny, (TY + 1) * by); ++j)
+            for (int i = TX * bx; i < std::min(nx, (TX + 1) * bx); ++i)
+              for (int q = 0; q < epc; ++q) {
+                const int e = ((k * ny + j) * nx + i) * epc + q;
+                t.ref_of[(size_t)T * epb + el] = e;
+                t.dev_of[e] = T * epb + el;
+                ++el;
+              }
+        t.count[T] = el;
+      }
+  for (int e = 0; e < m.K; ++e)
+    if (t.dev_of[e] < 0) throw std::runtime_error("tiling: element not placed");
+  return t;
+}
+
+FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivity &c, const Tiling &t) {
+  const int NF = m.NF, NFP = m.NFP, epb = t.epb, ND = m.dim;
+  if (m.NP > 255 || NFP > 255) throw std::runtime_error("jobs: node index does not fit 8 bits");
   FaceJobs J;
-  const int ntiles = (m.K + epb - 1) / epb;
-  J.job_off.assign(ntiles + 1, 0);
   std::vector<std::vector<int>> pats;
   std::vector<int> pat(NFP);
-  for (int T = 0; T < ntiles; ++T) {
-    const int e0 = T * epb, nE = std::min(epb, m.K - e0);
+  // per tile: job records (4 ints) and the partner node of each slot
+  std::vector<std::vector<int32_t>> tj(t.ntiles);
+  std::vector<std::vector<int32_t>> tn(t.ntiles);
+  for (int T = 0; T < t.ntiles; ++T) {
+    const int nE = t.count[T];
     for (int el = 0; el < nE; ++el)
       for (int lf = 0; lf < NF; ++lf) {
-        const int e = e0 + el;
+        const int e = t.ref_of[(size_t)T * epb + el];
         const int32_t *cn = &c.conn[((size_t)e * NF + lf) * 4];
         const int partner = cn[0], fid = cn[1], side = cn[2] & 1, bc = (cn[2] >> 1) & 7, pid = cn[2] >> 4;
-        const int pl = partner - e0;
+        const int pdev = t.dev_of[partner];
+        const int pl = pdev - T * epb;
         // A pair needs both elements to reference the face: after a "-"
         // boundary overlay on an axis whose "+" side stays periodic, the
         // old periodic face keeps its side-0 element only.
@@ -504,6 +577,7 @@ FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivit
         const bool pair = plf >= 0;
         if (pair && side == 1) continue;  // evaluated by the side-0 element
         int packed = el | (lf << 7);
+        for (int r = 0; r < NFP; ++r) pat[r] = 0;
         if (pair) {
           for (int r = 0; r < NFP; ++r) {
             const int pn = c.perm_tab[(size_t)pid * NFP + r];
@@ -523,21 +597,19 @@ FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivit
           if (id >= (1 << 13)) throw std::runtime_error("jobs: too many slot patterns");
           packed |= ((pl + 1) << 9) | (plf << 16) | (id << 18);
         }
-        J.jobs.push_back(packed);
-        J.jobs.push_back(cn[0]);
-        J.jobs.push_back(cn[1]);
-        J.jobs.push_back(cn[2]);
+        tj[T].insert(tj[T].end(), {packed, pdev, fid, cn[2]});
+        for (int r = 0; r < NFP; ++r) tn[T].push_back(c.perm_tab[(size_t)pid * NFP + r] | (pat[r] << 8));
       }
-    J.job_off[T + 1] = (int)(J.jobs.size() / 4);
+    J.maxj = std::max(J.maxj, (int)(tj[T].size() / 4));
   }
   J.npslot = (int)pats.size();
   // Every (element, face, slot) of a tile must be written exactly once.
   std::vector<int> hits((size_t)epb * NF * NFP);
-  for (int T = 0; T < ntiles; ++T) {
-    const int nE = std::min(epb, m.K - T * epb);
+  for (int T = 0; T < t.ntiles; ++T) {
+    const int nE = t.count[T];
     std::fill(hits.begin(), hits.end(), 0);
-    for (int j = J.job_off[T]; j < J.job_off[T + 1]; ++j) {
-      const int job = J.jobs[(size_t)j * 4], el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
+    for (size_t j = 0; j < tj[T].size() / 4; ++j) {
+      const int job = tj[T][j * 4], el = job & 127, lf = (job >> 7) & 3, pin = (job >> 9) & 127;
       if (el >= nE || (pin && pin - 1 >= nE)) throw std::runtime_error("jobs: element outside its tile");
       for (int r = 0; r < NFP; ++r) {
         ++hits[((size_t)el * NF + lf) * NFP + r];
@@ -546,6 +618,27 @@ FaceJobs build_face_jobs(const HostMesh &m, const RefElem &re, const Connectivit
     }
     for (int k = 0; k < nE * NF * NFP; ++k)
       if (hits[k] != 1) throw std::runtime_error("jobs: face slot not covered exactly once");
+  }
+  // fixed-stride device tables
+  const size_t slots = (size_t)t.ntiles * J.maxj;
+  J.jobs.assign(slots * 4, 0);
+  J.geo.assign(slots * 4, 0.0);
+  J.ipr.assign(slots * NFP, 0);
+  for (int T = 0; T < t.ntiles; ++T) {
+    const size_t nj = tj[T].size() / 4;
+    for (size_t j = 0; j < (size_t)J.maxj; ++j) {
+      const size_t s = (size_t)T * J.maxj + j;
+      if (j >= nj) {
+        J.jobs[s * 4 + 0] = -1;
+        J.jobs[s * 4 + 1] = T * epb;
+        continue;
+      }
+      for (int k = 0; k < 4; ++k) J.jobs[s * 4 + k] = tj[T][j * 4 + k];
+      const int fid = tj[T][j * 4 + 2];
+      for (int d = 0; d < ND; ++d) J.geo[s * 4 + d] = m.face_normal[(size_t)fid * ND + d];
+      J.geo[s * 4 + 3] = m.face_area[fid];
+      for (int r = 0; r < NFP; ++r) J.ipr[s * NFP + r] = (uint16_t)tn[T][j * NFP + r];
+    }
   }
   for (auto &p : pats) J.pslot_tab.insert(J.pslot_tab.end(), p.begin(), p.end());
   if (J.pslot_tab.empty()) J.pslot_tab.assign(NFP, 0);

@@ -42,6 +42,7 @@ struct Advection {
   MXM_HD void flux(const R *q, R *F) const {
     for (int d = 0; d < ND; ++d) F[d] = v[d] * q[0];
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
     R vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
     R a = rabs(vn);
@@ -290,13 +291,29 @@ struct Euler {
       for (int m = 0; m < 5; ++m) rf[m] = R(0.5) * (fl[m] + fr[m]) - R(0.5) * (apdq[m] - amdq[m]);
     }
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
-    Frame F = frame(n);
     R rl[5], rr[5], rf[5];
-    to_local(F, ql, rl);
-    to_local(F, qr, rr);
-    local_flux(rl, rr, rf);
-    to_world(F, rf, f);
+    if constexpr (ND == 2) {
+      // n = (nx, ny, 0): the frame (n, (-ny, nx, 0), (0, 0, 1)).  The flux
+      // is invariant under the choice of tangent basis (only |v_t| and the
+      // tangential components as vectors enter), so this equals the
+      // reference's constructed frame up to rounding.
+      const R nx = n[0], ny = n[1];
+      rl[0] = ql[0], rl[4] = ql[4], rr[0] = qr[0], rr[4] = qr[4];
+      rl[1] = nx * ql[1] + ny * ql[2], rl[2] = nx * ql[2] - ny * ql[1], rl[3] = ql[3];
+      rr[1] = nx * qr[1] + ny * qr[2], rr[2] = nx * qr[2] - ny * qr[1], rr[3] = qr[3];
+      local_flux(rl, rr, rf);
+      f[0] = rf[0], f[4] = rf[4], f[3] = rf[3];
+      f[1] = nx * rf[1] - ny * rf[2];
+      f[2] = ny * rf[1] + nx * rf[2];
+    } else {
+      Frame F = frame(n);
+      to_local(F, ql, rl);
+      to_local(F, qr, rr);
+      local_flux(rl, rr, rf);
+      to_world(F, rf, f);
+    }
   }
   // ghost states, src/euler.mojo:705-730: WALL reflects the normal
   // momentum, INFLOW is the stored state, OUTFLOW copies
@@ -374,6 +391,7 @@ struct Euler2D {
     F[0] = q[1], F[1] = q[1] * u + p, F[2] = q[1] * v, F[3] = u * (q[3] + p);
     F[4] = q[2], F[5] = q[2] * u, F[6] = q[2] * v + p, F[7] = v * (q[3] + p);
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
     const R nx = n[0], ny = n[1];
     R rl, ul, vl, pl, rr, ur, vr, pr;
@@ -449,6 +467,7 @@ struct SW2D {
     F[0] = q[1], F[1] = q[1] * u + p, F[2] = q[1] * v;
     F[3] = q[2], F[4] = q[2] * u, F[5] = q[2] * v + p;
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
     const R nx = n[0], ny = n[1];
     R FL[6], FR[6];
@@ -526,6 +545,7 @@ struct MHDGLM2D {
     if (disc < R(0)) disc = R(0);
     speed = rmax(sqrt(u * u + v * v) + sqrt(R(0.5) * (s + sqrt(disc))), ch);
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
     R FL[14], FR[14], sl, sr;
     state(ql, FL, sl);
@@ -575,6 +595,7 @@ struct Maxwell2D {
     F[0] = c2 * (-q[5] * ny), F[1] = c2 * (q[5] * nx), F[2] = c2 * (q[3] * ny - q[4] * nx);
     F[3] = ny * q[2], F[4] = -nx * q[2], F[5] = nx * q[1] - ny * q[0];
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
     R c2 = c * c, FL[6], FR[6];
     normal_flux(ql, c2, n[0], n[1], FL);
@@ -613,6 +634,7 @@ struct Advection2D {
     F[0] = vx * q[0];
     F[1] = vy * q[0];
   }
+  template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
     R vn = vx * n[0] + vy * n[1];
     f[0] = vn >= R(0) ? vn * ql[0] : vn * qr[0];

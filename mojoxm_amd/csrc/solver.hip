@@ -85,10 +85,16 @@ struct Solver {
   bool limiter = false;
   double eps = 0.1;
   DevBuf<double> q, q1, q2, mean;
-  DevBuf<double> invJ, inv_vol, fnormal, farea, Dt, Lt, w;
+  DevBuf<double> invJ, inv_vol, jgeo, Dt, Lt, w;
   DevBuf<int4> conn;
-  DevBuf<int> perm, fmap, job_off, pslot;
+  DevBuf<int> fmap, tile_count;
   DevBuf<int4> jobs;
+  DevBuf<unsigned short> ipr;
+  int maxj = 0;
+  // Device element order: tiles of lattice cells, each contiguous
+  // (slot T*epb + el); the host side of the ABI keeps the reference order.
+  Tiling til;
+  int Kd = 0;  // device element slots, padding included
   int ntiles = 0;
 
   ~Solver() {
@@ -99,27 +105,42 @@ struct Solver {
     dim = m.dim, P = m.P, NP = m.NP, NFP = m.NFP, NF = m.NF, K = m.K, Nf = m.Nf;
     Connectivity cn = build_connectivity(m, re);
     nperm = cn.nperm;
+    til = build_tiling(m, stage_epb(NP), NFP);
+    Kd = til.slots();
+    ntiles = til.ntiles;
     MXM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    const size_t nq = (size_t)K * NP * NC;
+    const size_t nq = (size_t)Kd * NP * NC;
     q.alloc(nq);
     q1.alloc(nq);
     q2.alloc(nq);
     MXM_HIP(hipMemset(q.p, 0, q.bytes()));
     MXM_HIP(hipMemset(q1.p, 0, q1.bytes()));
     MXM_HIP(hipMemset(q2.p, 0, q2.bytes()));
-    mean.alloc((size_t)K * NC);
-    invJ.upload(m.elem_invJ.data(), m.elem_invJ.size());
-    inv_vol.upload(m.elem_inv_vol.data(), m.elem_inv_vol.size());
-    fnormal.upload(m.face_normal.data(), m.face_normal.size());
-    farea.upload(m.face_area.data(), m.face_area.size());
-    conn.upload(reinterpret_cast<const int4 *>(cn.conn.data()), (size_t)K * NF);
-    perm.upload(cn.perm_tab.data(), cn.perm_tab.size());
-    const int epb = stage_epb(NP);
-    FaceJobs fj = build_face_jobs(m, re, cn, epb);
-    ntiles = (K + epb - 1) / epb;
-    job_off.upload(fj.job_off.data(), fj.job_off.size());
+    mean.alloc((size_t)Kd * NC);
+    // per-element tables in device order; a padding slot is its own
+    // neighbour with a zero state, which the limiter leaves unchanged
+    const int dd = dim * dim;
+    std::vector<double> ij((size_t)Kd * dd, 0.0), iv((size_t)Kd, 0.0);
+    std::vector<int4> cd((size_t)Kd * NF);
+    for (int sl = 0; sl < Kd; ++sl) {
+      const int e = til.ref_of[sl];
+      for (int lf = 0; lf < NF; ++lf) {
+        const int32_t *c = &cn.conn[((size_t)(e < 0 ? 0 : e) * NF + lf) * 4];
+        cd[(size_t)sl * NF + lf] = e < 0 ? int4{sl, 0, 0, 0} : int4{til.dev_of[c[0]], c[1], c[2], 0};
+      }
+      if (e < 0) continue;
+      for (int k = 0; k < dd; ++k) ij[(size_t)sl * dd + k] = m.elem_invJ[(size_t)e * dd + k];
+      iv[sl] = m.elem_inv_vol[e];
+    }
+    invJ.upload(ij.data(), ij.size());
+    inv_vol.upload(iv.data(), iv.size());
+    conn.upload(cd.data(), cd.size());
+    FaceJobs fj = build_face_jobs(m, re, cn, til);
+    maxj = fj.maxj;
+    tile_count.upload(til.count.data(), til.count.size());
     jobs.upload(reinterpret_cast<const int4 *>(fj.jobs.data()), fj.jobs.size() / 4);
-    pslot.upload(fj.pslot_tab.data(), fj.pslot_tab.size());
+    jgeo.upload(fj.geo.data(), fj.geo.size());
+    ipr.upload(fj.ipr.data(), fj.ipr.size());
     // transposed operators, node columns zero-padded as the kernel expects
     const int npad = stage_node_pad(NP, NC);
     std::vector<double> dt((size_t)dim * NP * npad, 0.0), lt((size_t)NF * NFP * npad, 0.0);
@@ -148,10 +169,10 @@ struct Solver {
     } else {
       a.q_in = q2.p, a.q_a = q.p, a.q_out = q.p, a.a = 1.0 / 3.0, a.b = 2.0 / 3.0, a.cdt = (2.0 / 3.0) * dt;
     }
-    a.K = K;
-    a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.fnormal = fnormal.p, a.farea = farea.p;
-    a.perm = perm.p, a.Dt = Dt.p, a.Lt = Lt.p, a.fmap = fmap.p;
-    a.job_off = job_off.p, a.jobs = jobs.p, a.pslot = pslot.p, a.ntiles = ntiles;
+    a.tile_count = tile_count.p;
+    a.invJ = invJ.p, a.inv_vol = inv_vol.p;
+    a.Dt = Dt.p, a.Lt = Lt.p, a.fmap = fmap.p;
+    a.jobs = jobs.p, a.jgeo = jgeo.p, a.ipr = ipr.p, a.maxj = maxj, a.ntiles = ntiles;
     a.post_comp = -1;
     a.post_scale = 1.0;
     // GLM psi damping q[psi] *= exp(-alpha_d dt) once per step after the
@@ -163,7 +184,7 @@ struct Solver {
       a.post_scale = std::exp(-prm[4] * dt);
     }
     MXM_HIP(launch(a, prm, stream));
-    if (limiter) MXM_HIP(launch_limiter(const_cast<double *>(a.q_out), mean.p, w.p, conn.p, K, NP, NC, NF, dim, eps, stream));
+    if (limiter) MXM_HIP(launch_limiter(a.q_out, mean.p, w.p, conn.p, Kd, NP, NC, NF, dim, eps, stream));
   }
 
   void step(double dt, int n) {
@@ -172,6 +193,16 @@ struct Solver {
   }
 
   long long dof() const { return (long long)K * NP * NC; }
+  // reference order [K][NP][NC] <-> device slots
+  void to_device(const double *h, double *d) const {
+    const size_t node = (size_t)NP * NC;
+    std::fill(d, d + (size_t)Kd * node, 0.0);
+    for (int e = 0; e < K; ++e) std::memcpy(d + (size_t)til.dev_of[e] * node, h + (size_t)e * node, node * sizeof(double));
+  }
+  void to_host(const double *d, double *h) const {
+    const size_t node = (size_t)NP * NC;
+    for (int e = 0; e < K; ++e) std::memcpy(h + (size_t)e * node, d + (size_t)til.dev_of[e] * node, node * sizeof(double));
+  }
   long long state_bytes_per_step() const { return 8LL * dof() * (long long)sizeof(double); }
 };
 
@@ -316,7 +347,7 @@ int mxm_mesh_create_3d(const mxm_mesh3d_desc *d, int P, void **mesh) {
     auto m = std::make_unique<HostMesh>(build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re));
     // the kernel-side tables are validated here too, so host-only callers
     // exercise the same checks a solver runs
-    build_face_jobs(*m, re, build_connectivity(*m, re), stage_epb(m->NP));
+    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->NP), m->NFP));
     *mesh = m.release();
     return MXM_OK;
   });
@@ -330,7 +361,7 @@ int mxm_mesh_create_2d(const mxm_mesh2d_desc *d, int P, void **mesh) {
     check_bc(d->bc, 4);
     RefElem re = build_refelem(2, P);
     auto m = std::make_unique<HostMesh>(build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re));
-    build_face_jobs(*m, re, build_connectivity(*m, re), stage_epb(m->NP));
+    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->NP), m->NFP));
     *mesh = m.release();
     return MXM_OK;
   });
@@ -403,8 +434,10 @@ int mxm_dg_upload_q(void *h, const double *qh, long long n) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!qh || n != s->dof()) throw ArgError("state length must be K*NP*NC");
+    std::vector<double> dev(s->q.n);
+    s->to_device(qh, dev.data());
     MXM_HIP(hipStreamSynchronize(s->stream));
-    MXM_HIP(hipMemcpy(s->q.p, qh, n * sizeof(double), hipMemcpyHostToDevice));
+    MXM_HIP(hipMemcpy(s->q.p, dev.data(), s->q.bytes(), hipMemcpyHostToDevice));
     return MXM_OK;
   });
 }
@@ -413,8 +446,10 @@ int mxm_dg_download_q(void *h, double *qh, long long n) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!qh || n != s->dof()) throw ArgError("state length must be K*NP*NC");
+    std::vector<double> dev(s->q.n);
     MXM_HIP(hipStreamSynchronize(s->stream));
-    MXM_HIP(hipMemcpy(qh, s->q.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    MXM_HIP(hipMemcpy(dev.data(), s->q.p, s->q.bytes(), hipMemcpyDeviceToHost));
+    s->to_host(dev.data(), qh);
     return MXM_OK;
   });
 }
@@ -423,9 +458,10 @@ int mxm_dg_download_component(void *h, int c, double *out, long long n) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!out || c < 0 || c >= s->NC || n != (long long)s->K * s->NP) throw ArgError("bad component download");
-    std::vector<double> all((size_t)s->dof());
+    std::vector<double> dev(s->q.n), all((size_t)s->dof());
     MXM_HIP(hipStreamSynchronize(s->stream));
-    MXM_HIP(hipMemcpy(all.data(), s->q.p, all.size() * sizeof(double), hipMemcpyDeviceToHost));
+    MXM_HIP(hipMemcpy(dev.data(), s->q.p, s->q.bytes(), hipMemcpyDeviceToHost));
+    s->to_host(dev.data(), all.data());
     for (long long k = 0; k < n; ++k) out[k] = all[(size_t)k * s->NC + c];
     return MXM_OK;
   });
@@ -512,8 +548,8 @@ int mxm_dg_memory_report(void *h, mxm_memory_report *o) {
     o->dg_operators = (long long)(s->Dt.bytes() + s->Lt.bytes() + s->w.bytes() + s->fmap.bytes());
     o->limiter = (long long)s->mean.bytes();
     o->mesh_connectivity = (long long)(s->invJ.bytes() + s->inv_vol.bytes() + s->conn.bytes() +
-                                       s->fnormal.bytes() + s->farea.bytes() + s->perm.bytes() +
-                                       s->job_off.bytes() + s->jobs.bytes() + s->pslot.bytes());
+                                       s->jgeo.bytes() + s->jobs.bytes() + s->ipr.bytes() +
+                                       s->tile_count.bytes());
     o->halo_device = 0;
     o->halo_pinned = 0;
     return MXM_OK;
@@ -526,24 +562,21 @@ int mxm_dg_time_stage_kernel(void *h, double dt, int nsteps, double *avg_us) {
     if (!avg_us || nsteps <= 0) throw ArgError("bad timing arguments");
     const bool lim = s->limiter;
     s->limiter = false;
-    std::vector<hipEvent_t> ev(6 * (size_t)nsteps);
+    // two events on the solver's stream around 3*nsteps back-to-back stage
+    // launches: the average includes the launch-to-launch gaps, as the
+    // stepping loop does
+    hipEvent_t ev[2];
     for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
+    MXM_HIP(hipEventRecord(ev[0], s->stream));
     for (int k = 0; k < nsteps; ++k)
-      for (int st = 0; st < 3; ++st) {
-        MXM_HIP(hipEventRecord(ev[(size_t)(k * 3 + st) * 2], s->stream));
-        s->stage(st, dt);
-        MXM_HIP(hipEventRecord(ev[(size_t)(k * 3 + st) * 2 + 1], s->stream));
-      }
-    MXM_HIP(hipStreamSynchronize(s->stream));
-    double total = 0.0;
-    for (int k = 0; k < 3 * nsteps; ++k) {
-      float ms = 0.f;
-      MXM_HIP(hipEventElapsedTime(&ms, ev[(size_t)k * 2], ev[(size_t)k * 2 + 1]));
-      total += ms;
-    }
+      for (int st = 0; st < 3; ++st) s->stage(st, dt);
+    MXM_HIP(hipEventRecord(ev[1], s->stream));
+    MXM_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0.f;
+    MXM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     for (auto &e : ev) (void)hipEventDestroy(e);
     s->limiter = lim;
-    *avg_us = total * 1e3 / (3.0 * nsteps);
+    *avg_us = (double)ms * 1e3 / (3.0 * nsteps);
     return MXM_OK;
   });
 }

@@ -44,20 +44,18 @@ struct StageArgs {
   const R *q_in, *q_a;
   R *q_out;
   R a, b, cdt;
-  int K;
-  const R *invJ;       // [K][ND*ND] rows = reference directions
-  const R *inv_vol;    // [K] 1/(6V) or 1/(2A)
-  const R *fnormal;    // [Nf][ND]
-  const R *farea;      // [Nf]
-  const int *perm;     // [nperm][NFP] partner node per reference face slot
+  const int *tile_count;  // [ntiles] elements in each tile (slots T*EPB ..)
+  const R *invJ;       // [slots][ND*ND] rows = reference directions
+  const R *inv_vol;    // [slots] 1/(6V) or 1/(2A)
   const R *Dt;         // [ND*NP (k,j)][NPAD (i)] transposed D_ref, zero padded
   const R *Lt;         // [NF*NFP (f,r)][NPAD (i)] transposed Lift_ref, zero padded
   const int *fmap;     // [NF][NFP] element node of each reference face slot
-  const int *job_off;  // [ntiles+1] face-flux jobs of each tile
-  const int4 *jobs;    // {el | lf<<7 | pin<<9 | plf<<16 | pslot<<18, partner,
-                       //  face id, side | bc<<1 | perm<<4}
-  const int *pslot;    // [npslot][NFP] partner reference slot per slot
-  int ntiles;          // tiles the job lists were built for
+  const int4 *jobs;    // [ntiles*maxj] {el | lf<<7 | pin<<9 | plf<<16 | pslot<<18
+                       //  (-1: padding), partner slot, face id, side | bc<<1}
+  const double *jgeo;  // [ntiles*maxj][4] face normal, area
+  const unsigned short *ipr;  // [ntiles*maxj*NFP] partner node | partner slot << 8
+  int maxj;            // job slots per tile
+  int ntiles;
   int post_comp;       // component scaled after the combine (GLM psi), -1 none
   R post_scale;
 };
@@ -99,27 +97,33 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 // global memory before the flux is evaluated.
 template <class R, int NC>
 struct FaceItem {
-  int packed, r, side, bc, pn, pl;
+  int packed, r, side, bc, pn, rs, pl;
   R n[3], area;
   R qp[NC];  // partner trace when the partner is outside the tile
 };
 
+// Every record of face item t of a tile is addressed from the tile index,
+// so all of them are in flight together; the partner trace (outside the
+// tile) is the only load that waits for another.
 template <int ND, int NP, int NFP, int NC, class R>
-__device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int j0, int t, int e0, int nE,
+__device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int tile, int t, int e0, int nE,
                                                FaceItem<R, NC> &it) {
   const int jb = t / NFP;
   it.r = t - jb * NFP;
-  const int4 jr = A.jobs[j0 + jb];
+  const size_t j = (size_t)tile * A.maxj + jb;
+  const int4 jr = A.jobs[j];
+  const double2 g0 = reinterpret_cast<const double2 *>(A.jgeo)[2 * j];
+  const double2 g1 = reinterpret_cast<const double2 *>(A.jgeo)[2 * j + 1];
+  const unsigned pr = A.ipr[(size_t)tile * A.maxj * NFP + t];
   it.packed = jr.x;
   it.side = jr.w & 1;
   it.bc = (jr.w >> 1) & 7;
-#pragma unroll
-  for (int d = 0; d < ND; ++d) it.n[d] = A.fnormal[(size_t)jr.z * ND + d];
-  if constexpr (ND == 2) it.n[2] = R(0);
-  it.area = A.farea[jr.z];
-  it.pn = A.perm[(jr.w >> 4) * NFP + it.r];
+  it.n[0] = R(g0.x), it.n[1] = R(g0.y), it.n[2] = R(g1.x);
+  it.area = R(g1.y);
+  it.pn = pr & 255;
+  it.rs = pr >> 8;
   it.pl = jr.y - e0;
-  const bool need = !(it.side == 0 && it.bc != 0);
+  const bool need = jr.x >= 0 && !(it.side == 0 && it.bc != 0);
   if (need && !(it.pl >= 0 && it.pl < nE)) {
     const R *g = A.q_in + ((size_t)jr.y * NP + it.pn) * NC;
 #pragma unroll
@@ -131,6 +135,7 @@ template <class Phys, int ND, int P, class R>
 __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys) {
   using S = Shape<ND, P>;
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
+  constexpr int TILE = EPB * NP * NC;  // values of one element tile
   // Contraction layout: the (element, component) columns of the tile are
   // spread over one or more waves; the block's waves split into GROUPS
   // groups that each produce a contiguous range of IH output nodes, so the
@@ -139,26 +144,30 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   constexpr int GSIZE = S::BLOCK / GROUPS;
   constexpr int NPAD = stage_node_pad(NP, NC);
   constexpr int IH = NPAD / GROUPS;
-  __shared__ R sQ[EPB * NP * NC];
-  __shared__ R sF[ND * NP * EPB * NC];  // [k][j][el][c]; later the rhs [el][i][c]
-  __shared__ R sS[NF * NFP * EPB * NC]; // [f*NFP + r][el][c]
-  __shared__ R sJ[EPB * ND * ND + EPB]; // invJ then inv_vol
+  constexpr int ITER = (EPB * NC + GSIZE - 1) / GSIZE;  // columns per thread, at most
+  // sW holds ND tile-sized planes.  Plane 0 first holds the q_in tile as
+  // [el][j][c]; after the face phase the planes hold the contravariant
+  // nodal fluxes as [k][j][el][c]; after the contraction plane 0 holds
+  // c dt (volume - lift) as [el][i][c].
+  __shared__ R sW[ND * TILE];
+  __shared__ R sS[NF * NFP * EPB * NC];  // [f*NFP + r][el][c]
+  __shared__ R sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
 
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int e0 = tile * EPB;
-  const int nE = min(EPB, A.K - e0);
+  const int nE = A.tile_count[tile];
   if (nE <= 0) return;
-  const int j0 = A.job_off[tile];
-  const int nitems = (A.job_off[tile + 1] - j0) * NFP;
+  const int nitems = A.maxj * NFP;  // padding items have packed < 0
+  // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
+  const bool node = tid < nE * NP;
 
   // ---- 1. first face item of this thread, then the tile ------------------
   FaceItem<R, NC> it;
-  if (tid < nitems) load_face_item<ND, NP, NFP, NC>(A, j0, tid, e0, nE, it);
-  // q_a of this thread's output node (EPB*NP <= 256: one node per thread)
+  if (tid < nitems) load_face_item<ND, NP, NFP, NC>(A, tile, tid, e0, nE, it);
   R qa[NC];
-  if (tid < nE * NP) {
+  if (node) {
     const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) qa[c] = g[c];
@@ -166,34 +175,19 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   {
     const R *src = A.q_in + (size_t)e0 * NP * NC;
     const int n = nE * NP * NC;
-    for (int t = tid; t < n; t += S::BLOCK) sQ[t] = src[t];
+    for (int t = tid; t < n; t += S::BLOCK) sW[t] = src[t];
     const R *gJ = A.invJ + (size_t)e0 * ND * ND;
     for (int t = tid; t < nE * ND * ND; t += S::BLOCK) sJ[t] = gJ[t];
     for (int t = tid; t < nE; t += S::BLOCK) sJ[EPB * ND * ND + t] = A.inv_vol[e0 + t];
     for (int t = tid; t < NF * NFP; t += S::BLOCK) sMap[t] = A.fmap[t];
   }
   __syncthreads();
+  const R *sQ = sW;  // the q_in tile, [el][j][c], until the flux planes are written
 
-  // ---- 2a. contravariant nodal fluxes ------------------------------------
-  for (int t = tid; t < nE * NP; t += S::BLOCK) {
-    const int el = t / NP, j = t - el * NP;
-    R F[ND * NC];
-    phys.template flux<ND>(&sQ[(el * NP + j) * NC], F);
-    const R *iJ = &sJ[el * ND * ND];
-#pragma unroll
-    for (int k = 0; k < ND; ++k)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        R v = R(0);
-#pragma unroll
-        for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * F[d * NC + c];
-        sF[((k * NP + j) * EPB + el) * NC + c] = v;
-      }
-  }
-
-  // ---- 2b. face fluxes ----------------------------------------------------
+  // ---- 2a. face fluxes ----------------------------------------------------
   for (int t = tid; t < nitems; t += S::BLOCK) {
-    if (t != tid) load_face_item<ND, NP, NFP, NC>(A, j0, t, e0, nE, it);
+    if (t != tid) load_face_item<ND, NP, NFP, NC>(A, tile, t, e0, nE, it);
+    if (it.packed < 0) continue;
     const int el = it.packed & 127, lf = (it.packed >> 7) & 3, pin = (it.packed >> 9) & 127;
     R qm[NC], qo[NC], ql[NC], qr[NC];
     const int mine = (el * NP + sMap[lf * NFP + it.r]) * NC;
@@ -216,81 +210,113 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
 #pragma unroll
       for (int c = 0; c < NC; ++c) qr[c] = it.side == 0 ? qo[c] : qm[c];
     R Fs[NC];
-    phys.numerical_flux(ql, qr, it.n, Fs);
+    phys.template numerical_flux<ND>(ql, qr, it.n, Fs);
     const R s = it.side == 0 ? it.area : -it.area;
 #pragma unroll
     for (int c = 0; c < NC; ++c) sS[((lf * NFP + it.r) * EPB + el) * NC + c] = s * Fs[c];
     if (pin) {
-      const int plf = (it.packed >> 16) & 3, rs = A.pslot[(it.packed >> 18) * NFP + it.r];
+      const int plf = (it.packed >> 16) & 3, rs = it.rs;
 #pragma unroll
       for (int c = 0; c < NC; ++c) sS[((plf * NFP + rs) * EPB + (pin - 1)) * NC + c] = -(s * Fs[c]);
     }
   }
+
+  // ---- 2b. nodal flux of the own node, and the RK terms of q_in ----------
+  // qa becomes a q_a + b q_in + c dt S(q_in); the flux, mapped to
+  // reference directions by invJ, waits in registers until every read of
+  // the q_in tile is done.
+  R Fk[ND * NC];
+  if (node) {
+    const int el = tid / NP;
+    const R *q = &sQ[tid * NC];
+    R F[ND * NC];
+    phys.template flux<ND>(q, F);
+    const R *iJ = &sJ[el * ND * ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        R v = R(0);
+#pragma unroll
+        for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * F[d * NC + c];
+        Fk[k * NC + c] = v;
+      }
+    R src[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) src[c] = R(0);
+    if constexpr (Phys::kSource) {
+      if (phys.source_active()) phys.source(q, src);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qa[c] = A.a * qa[c] + A.b * q[c] + A.cdt * src[c];
+  }
+  __syncthreads();
+  if (node) {
+    const int el = tid / NP, j = tid - el * NP;
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sW[k * TILE + (j * EPB + el) * NC + c] = Fk[k * NC + c];
+  }
   __syncthreads();
 
   // ---- 3a. contractions per (element, component) column ------------------
-  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr,
-  // written into sQ (no longer read after the face phase) as [el][i][c].
-  {
-    const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
-    const int i0 = grp * IH;
-    for (int p = tid - grp * GSIZE; p < nE * NC; p += GSIZE) {
-      const int el = p / NC, c = p - el * NC;
-      R acc[IH];
+  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr
+  const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
+  const int i0 = grp * IH;
+  const int lane = tid - grp * GSIZE;
+  R acc[ITER][IH];
 #pragma unroll
-      for (int ii = 0; ii < IH; ++ii) acc[ii] = R(0);
+  for (int m = 0; m < ITER; ++m) {
+    const int p = lane + m * GSIZE;
+#pragma unroll
+    for (int ii = 0; ii < IH; ++ii) acc[m][ii] = R(0);
+    if (p < nE * NC) {
+      const int el = p / NC;
 #pragma unroll 4
       for (int fr = 0; fr < NF * NFP; ++fr) {
         const R sv = sS[fr * EPB * NC + p];
         const R *L = A.Lt + fr * NPAD + i0;
 #pragma unroll
-        for (int ii = 0; ii < IH; ++ii) acc[ii] += L[ii] * sv;
+        for (int ii = 0; ii < IH; ++ii) acc[m][ii] += L[ii] * sv;
       }
       const R niv = -sJ[EPB * ND * ND + el];
 #pragma unroll
-      for (int ii = 0; ii < IH; ++ii) acc[ii] *= niv;
+      for (int ii = 0; ii < IH; ++ii) acc[m][ii] *= niv;
 #pragma unroll 4
       for (int kj = 0; kj < ND * NP; ++kj) {
-        const R fv = sF[kj * EPB * NC + p];
+        const R fv = sW[kj * EPB * NC + p];
         const R *D = A.Dt + kj * NPAD + i0;
 #pragma unroll
-        for (int ii = 0; ii < IH; ++ii) acc[ii] += D[ii] * fv;
+        for (int ii = 0; ii < IH; ++ii) acc[m][ii] += D[ii] * fv;
       }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < ITER; ++m) {
+    const int p = lane + m * GSIZE;
+    if (p < nE * NC) {
+      const int el = p / NC, c = p - el * NC;
 #pragma unroll
       for (int ii = 0; ii < IH; ++ii)
-        if (i0 + ii < NP) {
-          // the slot holds q_in of this (node, component): fold in b q_in
-          R &slot = sQ[(el * NP + i0 + ii) * NC + c];
-          slot = A.b * slot + A.cdt * acc[ii];
-        }
+        if (i0 + ii < NP) sW[(el * NP + i0 + ii) * NC + c] = A.cdt * acc[m][ii];
     }
   }
   __syncthreads();
 
-  // ---- 3b. source, RK combine, limit, store per (element, node) ---------
-  if (tid < nE * NP) {
-    const int t = tid;
-    const size_t g = (size_t)e0 * NP * NC + (size_t)t * NC;
-    R src[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) src[c] = R(0);
-    if constexpr (Phys::kSource) {
-      if (phys.source_active()) {
-        R qi[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) qi[c] = A.q_in[g + c];
-        phys.source(qi, src);
-      }
-    }
+  // ---- 3b. RK combine, limit, store per (element, node) ------------------
+  if (node) {
     R out[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) out[c] = A.a * qa[c] + sQ[t * NC + c] + A.cdt * src[c];
+    for (int c = 0; c < NC; ++c) out[c] = qa[c] + sW[tid * NC + c];
     if constexpr (Phys::kLimit) phys.limit(out);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       if (c == A.post_comp) out[c] *= A.post_scale;
+    R *g = A.q_out + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) A.q_out[g + c] = out[c];
+    for (int c = 0; c < NC; ++c) g[c] = out[c];
   }
 }
 
@@ -298,9 +324,8 @@ template <class Phys, int ND, int P>
 hipError_t launch_stage(const StageArgs<double> &a, const double *prm, hipStream_t s) {
   using S = Shape<ND, P>;
   const Phys ph = Phys::from(prm);
-  const int nb = (a.K + S::EPB - 1) / S::EPB;
+  const int nb = a.ntiles;
   if (nb <= 0) return hipSuccess;
-  if (nb != a.ntiles) return hipErrorInvalidValue;  // job lists built for another tiling
   hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, double>), dim3(nb), dim3(S::BLOCK), 0, s, a, ph);
   return hipGetLastError();
 }
