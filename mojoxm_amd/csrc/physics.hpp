@@ -202,18 +202,6 @@ struct Euler {
       apdq[m] = t - amdq[m];
     }
   }
-  MXM_HD static void split(const R *s, const R *wave, R *amdq, R *apdq) {
-    for (int m = 0; m < 5; ++m) {
-      amdq[m] = R(0);
-      apdq[m] = R(0);
-      for (int w = 0; w < 3; ++w) {
-        if (s[w] < R(0))
-          amdq[m] += s[w] * wave[m * 3 + w];
-        else
-          apdq[m] += s[w] * wave[m * 3 + w];
-      }
-    }
-  }
   // face-aligned flux (rl, rr already rotated); src/euler.mojo:612-683
   MXM_HD void local_flux(const R *rl, const R *rr, R *rf) const {
     const Prim L = prim(rl), Rt = prim(rr);
@@ -227,7 +215,8 @@ struct Euler {
       rf[3] = R(0.5) * (rl[3] * ul + rr[3] * ur + cm * (rl[3] - rr[3]));
       rf[4] = R(0.5) * ((rl[4] + pl) * ul + (rr[4] + pr) * ur + cm * (rl[4] - rr[4]));
     } else {
-      R s[3], wave[15], amdq[5], apdq[5];
+      R s[3], wave[15];
+      [[maybe_unused]] R amdq[5], apdq[5];
       R ra[5];
       roe_avg(rl, rr, L, Rt, ra);
       if constexpr (FLUX == 1) {
@@ -269,11 +258,15 @@ struct Euler {
           // HLLEC, src/euler.mojo:415-485
           const R Al = dl * (s[0] - ul), Ar = dr * (s[2] - ur);
           s[1] = (pr - pl + Al * ul - Ar * ur) / (Al - Ar);
-          const R lm = Al / (s[0] - s[1]), rm = Ar / (s[2] - s[1]);
+          // two reciprocals for the four quotients Al/(s0-s1), Ar/(s2-s1),
+          // pl/Al, pr/Ar
+          const R d0 = s[0] - s[1], d2 = s[2] - s[1];
+          const R i02 = R(1) / (d0 * d2), iA = R(1) / (Al * Ar);
+          const R lm = Al * d2 * i02, rm = Ar * d0 * i02;
           R hl[5] = {lm, lm * s[1], lm * L.u[1], lm * L.u[2],
-                     lm * (rl[4] * L.ir + (s[1] - ul) * (s[1] + pl / Al))};
+                     lm * (rl[4] * L.ir + (s[1] - ul) * (s[1] + pl * Ar * iA))};
           R hr[5] = {rm, rm * s[1], rm * Rt.u[1], rm * Rt.u[2],
-                     rm * (rr[4] * Rt.ir + (s[1] - ur) * (s[1] + pr / Ar))};
+                     rm * (rr[4] * Rt.ir + (s[1] - ur) * (s[1] + pr * Al * iA))};
           for (int m = 0; m < 5; ++m) {
             wave[m * 3] = hl[m] - rl[m];
             wave[m * 3 + 1] = hr[m] - hl[m];
@@ -281,14 +274,18 @@ struct Euler {
           }
         }
       }
-      if constexpr (efix)
-        entropy_fixed(rl, rr, s, wave, amdq, apdq);
-      else
-        split(s, wave, amdq, apdq);
       R fl[5], fr[5];
       flux1(rl, L, fl);
       flux1(rr, Rt, fr);
-      for (int m = 0; m < 5; ++m) rf[m] = R(0.5) * (fl[m] + fr[m]) - R(0.5) * (apdq[m] - amdq[m]);
+      if constexpr (efix) {
+        entropy_fixed(rl, rr, s, wave, amdq, apdq);
+        for (int m = 0; m < 5; ++m) rf[m] = R(0.5) * (fl[m] + fr[m]) - R(0.5) * (apdq[m] - amdq[m]);
+      } else {
+        // apdq - amdq = sum_w |s_w| wave_w (src/euler.mojo:184-199, 495-520)
+        const R a0 = rabs(s[0]), a1 = rabs(s[1]), a2 = rabs(s[2]);
+        for (int m = 0; m < 5; ++m)
+          rf[m] = R(0.5) * (fl[m] + fr[m]) - R(0.5) * (a0 * wave[m * 3] + a1 * wave[m * 3 + 1] + a2 * wave[m * 3 + 2]);
+      }
     }
   }
   template <int ND = 3>

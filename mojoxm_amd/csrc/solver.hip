@@ -105,7 +105,7 @@ struct Solver {
     dim = m.dim, P = m.P, NP = m.NP, NFP = m.NFP, NF = m.NF, K = m.K, Nf = m.Nf;
     Connectivity cn = build_connectivity(m, re);
     nperm = cn.nperm;
-    til = build_tiling(m, stage_epb(NP), NFP);
+    til = build_tiling(m, stage_epb(dim, NP), NFP);
     Kd = til.slots();
     ntiles = til.ntiles;
     MXM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -142,7 +142,7 @@ struct Solver {
     jgeo.upload(fj.geo.data(), fj.geo.size());
     ipr.upload(fj.ipr.data(), fj.ipr.size());
     // transposed operators, node columns zero-padded as the kernel expects
-    const int npad = stage_node_pad(NP, NC);
+    const int npad = stage_node_pad(dim, NP, NC);
     std::vector<double> dt((size_t)dim * NP * npad, 0.0), lt((size_t)NF * NFP * npad, 0.0);
     for (int d = 0; d < dim; ++d)
       for (int i = 0; i < NP; ++i)
@@ -347,7 +347,7 @@ int mxm_mesh_create_3d(const mxm_mesh3d_desc *d, int P, void **mesh) {
     auto m = std::make_unique<HostMesh>(build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re));
     // the kernel-side tables are validated here too, so host-only callers
     // exercise the same checks a solver runs
-    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->NP), m->NFP));
+    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->dim, m->NP), m->NFP));
     *mesh = m.release();
     return MXM_OK;
   });
@@ -361,7 +361,7 @@ int mxm_mesh_create_2d(const mxm_mesh2d_desc *d, int P, void **mesh) {
     check_bc(d->bc, 4);
     RefElem re = build_refelem(2, P);
     auto m = std::make_unique<HostMesh>(build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re));
-    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->NP), m->NFP));
+    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->dim, m->NP), m->NFP));
     *mesh = m.release();
     return MXM_OK;
   });

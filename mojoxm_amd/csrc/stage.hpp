@@ -60,20 +60,24 @@ struct StageArgs {
   R post_scale;
 };
 
-// Elements per 256-thread block for NP nodes per element; the host builds
-// the face job lists with the same tiling.
-constexpr int stage_epb(int NP) { return 256 / NP; }
+// Elements per 256-thread block for NP nodes per element, rounded down to
+// whole lattice cells (2 triangles, 6 tets) when a cell fits: tiles are
+// blocks of cells, so the LDS is sized for what a tile can hold.  The host
+// builds the tiles with the same value.
+constexpr int stage_epb(int ND, int NP) {
+  return 256 / NP >= (ND == 3 ? 6 : 2) ? 256 / NP - (256 / NP) % (ND == 3 ? 6 : 2) : 256 / NP;
+}
 
 // Contraction layout: the tile's (element, component) columns span
 // ceil(EPB*NC/64) waves; the block's 4 waves form `groups` groups, each
 // producing `IH` consecutive output nodes, so the operator entries a wave
 // reads are wave-uniform.  The transposed operators are stored with
 // groups*IH (>= NP) zero-padded node columns.
-constexpr int stage_groups(int NP, int NC) {
-  return (stage_epb(NP) * NC + 63) / 64 >= 4 ? 1 : 4 / ((stage_epb(NP) * NC + 63) / 64);
+constexpr int stage_groups(int ND, int NP, int NC) {
+  return (stage_epb(ND, NP) * NC + 63) / 64 >= 4 ? 1 : 4 / ((stage_epb(ND, NP) * NC + 63) / 64);
 }
-constexpr int stage_node_pad(int NP, int NC) {
-  return stage_groups(NP, NC) * ((NP + stage_groups(NP, NC) - 1) / stage_groups(NP, NC));
+constexpr int stage_node_pad(int ND, int NP, int NC) {
+  return stage_groups(ND, NP, NC) * ((NP + stage_groups(ND, NP, NC) - 1) / stage_groups(ND, NP, NC));
 }
 
 template <int ND, int P>
@@ -82,7 +86,7 @@ struct Shape {
   static constexpr int NFP = ND == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
   static constexpr int NF = ND + 1;
   static constexpr int BLOCK = 256;
-  static constexpr int EPB = stage_epb(NP);
+  static constexpr int EPB = stage_epb(ND, NP);
 };
 
 // Bijective map from the hardware block index (dispatched round-robin over
@@ -140,9 +144,9 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   // spread over one or more waves; the block's waves split into GROUPS
   // groups that each produce a contiguous range of IH output nodes, so the
   // operator entries a wave needs are wave-uniform (scalar loads).
-  constexpr int GROUPS = stage_groups(NP, NC);
+  constexpr int GROUPS = stage_groups(ND, NP, NC);
   constexpr int GSIZE = S::BLOCK / GROUPS;
-  constexpr int NPAD = stage_node_pad(NP, NC);
+  constexpr int NPAD = stage_node_pad(ND, NP, NC);
   constexpr int IH = NPAD / GROUPS;
   constexpr int ITER = (EPB * NC + GSIZE - 1) / GSIZE;  // columns per thread, at most
   // sW holds ND tile-sized planes.  Plane 0 first holds the q_in tile as
@@ -166,12 +170,6 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   // ---- 1. first face item of this thread, then the tile ------------------
   FaceItem<R, NC> it;
   if (tid < nitems) load_face_item<ND, NP, NFP, NC>(A, tile, tid, e0, nE, it);
-  R qa[NC];
-  if (node) {
-    const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) qa[c] = g[c];
-  }
   {
     const R *src = A.q_in + (size_t)e0 * NP * NC;
     const int n = nE * NP * NC;
@@ -219,6 +217,15 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
 #pragma unroll
       for (int c = 0; c < NC; ++c) sS[((plf * NFP + rs) * EPB + (pin - 1)) * NC + c] = -(s * Fs[c]);
     }
+  }
+
+  // q_a of the own node: loaded only now, so its registers are not held
+  // through the face phase (the register peak)
+  R qa[NC];
+  if (node) {
+    const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qa[c] = g[c];
   }
 
   // ---- 2b. nodal flux of the own node, and the RK terms of q_in ----------
