@@ -9,11 +9,13 @@ mesh, state resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints one JSON line: DOF-stage-updates/s of the plain FP64 kernels, the
-L2 error of the T=2 state against the analytic vortex and against the CPU
-FP64 oracle (bounded sample), a roofline record for the stage kernel
-(algorithmic bytes / HIP-event kernel time) and the oracle's CPU
-throughput on the host cores as a baseline.
+Prints one JSON line.  `value` is the DOF-stage-updates/s of the FP32-MFMA
+form; the plain FP64 form is timed beside it (config.fp64, the speedup and
+roofline_fp64).  Both forms state their relative L2 and Linf error against
+the CPU FP64 oracle (bounded sample of the same run) and their L2 error
+against the analytic vortex at T=2; the FP32 form also against the FP64
+form at T=2.  roofline: algorithmic bytes / HIP-event kernel time for the
+stage kernel.  cpu_baseline: the oracle's throughput on the host cores.
 """
 from __future__ import annotations
 
@@ -33,19 +35,23 @@ HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 METRIC = "DOF-stage-updates/s FP32-MFMA vs FP64 @ stated L2; batched solves/s + p99"
 
 
-# Summary of the rocprofv3 counter passes over this workload's stage kernel;
-# its "traffic_bytes_per_launch:" line gives roofline.traffic.
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "dg_stage_kernel_euler_vortex_2d_p3.md")
+# Summaries of the rocprofv3 counter passes over this workload's stage
+# kernels (one per arithmetic form); their "traffic_bytes_per_launch:" line
+# gives roofline.traffic.
+TRAFFIC_PROFILES = {
+    "fp64": os.path.join(ROOT, "profiles", "dg_stage_kernel_euler_vortex_2d_p3.md"),
+    "fp32_mfma": os.path.join(ROOT, "profiles", "dg_stage_kernel_fp32_mfma_euler_vortex_2d_p3.md"),
+}
 
 
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def measured_traffic():
+def measured_traffic(form):
     """HBM bytes per stage launch from the committed counter summary, or None."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILES[form]) as f:
             m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
     except OSError:
         return None
@@ -68,6 +74,85 @@ def copy_calibration() -> int:
     return 0
 
 
+def batched_leg(M, args):
+    """Seeded ensemble of 2D GLM-MHD Alfven waves (P=3, NC=7, n log-uniform
+    in [16, 96], T=1), all instances in one batch (FP32-MFMA), against the
+    same instances solved one at a time (bounded, evenly spaced sample)."""
+    ens = M.problems.alfven_ensemble(count=args.ensemble)
+    q0 = {}
+    for m in ens:
+        mesh = M.LocalMesh2D(*m.n, *m.L, P=m.P)
+        q0[m.index] = m.ic(mesh.elem_node_xyz).astype(np.float32).astype(np.float64)
+    work = sum(3.0 * 2 * m.n[0] * m.n[1] * 10 * 7 * m.nsteps for m in ens)
+    finish = {}
+    with M.batch.Batch(M.PREC_FP32_MFMA) as b:
+        ids = {b.add(m.n, m.L, m.physics, m.P, q0[m.index], m.dt, m.nsteps): m for m in ens}
+        b.step(0)  # every state resident in HBM before the clock starts
+        b.synchronize()
+        t0 = time.perf_counter()
+        while len(finish) < len(ens):
+            b.step(50)
+            now = time.perf_counter() - t0
+            for i in b.poll():
+                finish[i] = now
+        wall = time.perf_counter() - t0
+        exact_err = []
+        for i, m in ids.items():
+            mesh = M.LocalMesh2D(*m.n, *m.L, P=m.P)
+            exact_err.append(rel_l2(b.get_state(i), m.exact(mesh.elem_node_xyz, m.dt * m.nsteps)))
+        mem = b.memory()
+    # one at a time, FP32-MFMA and FP64, on an evenly spaced sample
+    step = max(1, len(ens) // args.solo_sample)
+    sample = ens[::step][: args.solo_sample]
+    solo, err64 = 0.0, []
+    for m in sample:
+        outs = {}
+        for prec in (M.PREC_FP32_MFMA, M.PREC_FP64):
+            with M.Solver(m.n, m.L, m.physics, P=m.P, precision=prec) as s:
+                s.set_state(q0[m.index])
+                s.synchronize()
+                t = time.perf_counter()
+                s.step_ssprk3(m.dt, m.nsteps)
+                s.synchronize()
+                if prec == M.PREC_FP32_MFMA:
+                    solo += time.perf_counter() - t
+                outs[prec] = s.get_state()
+        err64.append(rel_l2(outs[M.PREC_FP32_MFMA], outs[M.PREC_FP64]))
+    solo_rate = len(sample) / solo
+    lat = sorted(finish.values())
+    return {"workload": f"{len(ens)} GLM-MHD Alfven-wave instances (seed 0), P=3, NC=7, n in [16,96], T=1",
+            "precision": "fp32_mfma", "solves": len(ens), "wall_s": wall, "solves_per_s": len(ens) / wall,
+            "dof_stage_updates_per_s": work / wall, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "device_bytes": mem,
+            "one_at_a_time_solves_per_s": solo_rate,
+            "one_at_a_time_sample": f"{len(sample)} evenly spaced instances, each alone with its own Solver",
+            "speedup_vs_one_at_a_time": (len(ens) / wall) / solo_rate,
+            "l2_vs_linear_wave_mean": float(np.mean(exact_err)), "l2_vs_linear_wave_max": float(np.max(exact_err)),
+            "l2_fp32_vs_fp64_sample_max": float(np.max(err64))}
+
+
+def serving_leg(M, args):
+    """Seeded Poisson stream (dam break / Sod / PEC cavity, n in [16,256],
+    P in {2,3,4}, T in [0.1,1]) under continuous batching (FP32-MFMA) with
+    an HBM budget, against one-at-a-time FIFO solving of the same stream."""
+    reqs = M.problems.serving_stream(count=args.requests, rate=args.rate)
+    states = {r.index: M.serve.initial_state(r).astype(np.float32).astype(np.float64) for r in reqs}
+    budget = 16 << 30
+    rep = M.serve.serve(reqs, M.PREC_FP32_MFMA, budget=budget, chunk=16, states=states)
+    secs, _ = M.serve.one_at_a_time(reqs, M.PREC_FP32_MFMA, states=states)
+    fifo = M.serve.fifo_latencies(reqs, secs)
+    fifo_makespan = max(r.arrival + l for r, l in zip(sorted(reqs, key=lambda r: r.arrival), fifo)) - \
+        min(r.arrival for r in reqs)
+    return {"workload": f"{len(reqs)} requests, Poisson {args.rate}/s (seed 0): SW dam break, Euler Sod, "
+                        "Maxwell PEC cavity; n in [16,256], P in {2,3,4}, T in [0.1,1]",
+            "precision": "fp32_mfma", "hbm_budget_bytes": budget, "peak_bytes": rep.peak_bytes,
+            "solves_per_s": rep.solves_per_s, "p50_s": rep.pct(50), "p99_s": rep.pct(99),
+            "one_at_a_time_solves_per_s": len(reqs) / fifo_makespan,
+            "one_at_a_time_p50_s": float(np.percentile(fifo, 50)),
+            "one_at_a_time_p99_s": float(np.percentile(fifo, 99)),
+            "speedup_vs_one_at_a_time": rep.solves_per_s / (len(reqs) / fifo_makespan)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +161,12 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=480, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the full run to T=2 (for profiling)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--ensemble", type=int, default=512, help="instances of the batched ensemble")
+    ap.add_argument("--solo-sample", type=int, default=32, help="ensemble instances also solved one at a time")
+    ap.add_argument("--requests", type=int, default=16, help="requests of the serving stream")
+    ap.add_argument("--rate", type=float, default=8.0, help="serving arrival rate, requests/s")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
@@ -92,38 +183,50 @@ def main() -> int:
     K_el = pb.n[0] * pb.n[1] * 2
     mesh = M.LocalMesh2D(*pb.n, *pb.L, P=pb.P)
     q0 = pb.ic(mesh.elem_node_xyz)
-    solver = M.Solver(pb.n, pb.L, pb.physics, P=pb.P)
-    U = solver.dof_count()  # K*NP*NC DOF per stage
+    # both forms start from the same FP32-representable state
+    q0 = q0.astype(np.float32).astype(np.float64)
+    exact = pb.exact(mesh.elem_node_xyz, pb.t_final)
 
-    # ---- timed region: K steps with the state resident in HBM ------------
-    solver.set_state(q0)
-    solver.step_ssprk3(pb.dt, args.warmup)
-    solver.synchronize()
-    t0 = time.perf_counter()
-    solver.step_ssprk3(pb.dt, args.steps)
-    solver.synchronize()
-    wall = time.perf_counter() - t0
-    value = 3.0 * U * args.steps / wall
-
-    # ---- stage-kernel duration from HIP events on the solver's stream ----
-    solver.set_state(q0)
-    avg_us = solver.time_stage_kernel(pb.dt, 50)
-    bytes_per_launch = 8.0 / 3.0 * U * 8  # 8 state touches per step of U FP64 values, per stage
-    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
-
-    # ---- accuracy of the full run to T=2 ----------------------------------
-    l2_exact = l2_exact_rho = None
-    if not args.no_accuracy:
+    forms = {"fp32_mfma": (M.PREC_FP32_MFMA, 4), "fp64": (M.PREC_FP64, 8)}
+    res, finals = {}, {}
+    for name, (prec, word) in forms.items():
+        solver = M.Solver(pb.n, pb.L, pb.physics, P=pb.P, precision=prec)
+        U = solver.dof_count()  # K*NP*NC DOF per stage
+        # ---- timed region: K steps with the state resident in HBM --------
         solver.set_state(q0)
-        solver.step_ssprk3(pb.dt, pb.nsteps)
-        q_end = solver.get_state()
-        exact = pb.exact(mesh.elem_node_xyz, pb.t_final)
-        l2_exact = rel_l2(q_end, exact)
-        l2_exact_rho = rel_l2(q_end[..., 0], exact[..., 0])
+        solver.step_ssprk3(pb.dt, args.warmup)
+        solver.synchronize()
+        t0 = time.perf_counter()
+        solver.step_ssprk3(pb.dt, args.steps)
+        solver.synchronize()
+        wall = time.perf_counter() - t0
+        # ---- stage-kernel duration from HIP events on the solver's stream
+        solver.set_state(q0)
+        avg_us = solver.time_stage_kernel(pb.dt, 50)
+        bytes_per_launch = 8.0 / 3.0 * U * word  # 8 state touches per step of U values, per stage
+        achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+        r = {"value": 3.0 * U * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name),
+                          "kernel_avg_us": avg_us, "algorithmic_bytes_per_launch": bytes_per_launch}}
+        # ---- accuracy of the full run to T=2 ------------------------------
+        if not args.no_accuracy:
+            solver.set_state(q0)
+            solver.step_ssprk3(pb.dt, pb.nsteps)
+            finals[name] = solver.get_state()
+            r["l2_vs_exact_T2"] = rel_l2(finals[name], exact)
+            r["l2_vs_exact_T2_density"] = rel_l2(finals[name][..., 0], exact[..., 0])
+        res[name] = (r, solver)
+
+    # FP32 form against the FP64 form at T=2 (the FP64 form is tied to the
+    # oracle at 1e-12 on the CPU sample below)
+    if finals:
+        d = finals["fp32_mfma"] - finals["fp64"]
+        res["fp32_mfma"][0]["l2_vs_fp64_T2"] = rel_l2(finals["fp32_mfma"], finals["fp64"])
+        res["fp32_mfma"][0]["linf_vs_fp64_T2"] = float(np.abs(d).max() / np.abs(finals["fp64"]).max())
 
     # ---- CPU baseline: the oracle on a bounded sample of the same run ----
     cpu = None
-    l2_oracle = None
     if not args.no_cpu:
         from oracle import oracle as O
 
@@ -134,38 +237,53 @@ def main() -> int:
         c0 = time.perf_counter()
         q_cpu = om.step(pb.physics.phys_id, pb.physics.params(), q0, pb.dt, n_cpu)
         c_wall = time.perf_counter() - c0
-        solver.set_state(q0)
-        solver.step_ssprk3(pb.dt, n_cpu)
-        l2_oracle = rel_l2(solver.get_state(), q_cpu)
+        for name, (r, solver) in res.items():
+            solver.set_state(q0)
+            solver.step_ssprk3(pb.dt, n_cpu)
+            q_gpu = solver.get_state()
+            r["l2_vs_oracle_sample"] = rel_l2(q_gpu, q_cpu)
+            r["linf_vs_oracle_sample"] = float(np.abs(q_gpu - q_cpu).max() / np.abs(q_cpu).max())
+        U = res["fp64"][1].dof_count()
         cpu = {"value": 3.0 * U * n_cpu / c_wall, "unit": "DOF-stage-updates/s", "cores": threads,
                "kind": "port",
                "sample": f"{n_cpu} of {pb.nsteps} SSPRK3 steps of the same 128^2 P=3 vortex run "
                          f"({c_wall:.1f} s on {threads} OpenMP threads)"}
-    solver.close()
+    U = res["fp64"][1].dof_count()
+    for _, solver in res.values():
+        solver.close()
 
+    batched = batched_leg(M, args) if not args.no_batch else None
+    serving = serving_leg(M, args) if not args.no_serve else None
+
+    r32, r64 = res["fp32_mfma"][0], res["fp64"][0]
+    fp64 = {k: v for k, v in r64.items() if k not in ("roofline",)}
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r32["value"],
         "unit": "DOF-stage-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": r32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {
             "workload": "euler_vortex_2d: 128x128x2 tris, P=3, NC=5, HLLEC, CFL 0.15, T=2",
-            "kernel": "plain FP64 fused stage kernel",
-            "elements": K_el, "dof_per_stage": U, "dt": pb.dt, "steps_to_T": pb.nsteps,
-            "l2_vs_exact_T2": l2_exact, "l2_vs_exact_T2_density": l2_exact_rho,
-            "l2_vs_oracle": l2_oracle,
+            "kernel": "FP32-MFMA fused stage kernel (FP32 state and fluxes, volume+LIFT on "
+                      "V_MFMA_F32_16X16X4F32, FP64 geometry, combine and positivity)",
+            "elements": K_el, "dof_per_stage": U, "dt": pb.dt,
+            "steps_to_T": pb.nsteps,
+            "fp32_mfma": {k: v for k, v in r32.items() if k not in ("roofline", "value", "ms_per_step")},
+            "fp64": fp64,
+            "speedup_fp32_mfma_vs_fp64": r32["value"] / r64["value"],
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel_avg_us": avg_us, "algorithmic_bytes_per_launch": bytes_per_launch},
+        "batched": batched,
+        "serving": serving,
+        "roofline": r32["roofline"],
+        "roofline_fp64": r64["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -70,7 +70,11 @@ extern "C" {
 #define MXM_PRM_MAX 16
 
 /* Arithmetic of the stage kernels. */
-#define MXM_PREC_FP64 0 /* plain FP64 state, geometry and operators */
+#define MXM_PREC_FP64 0      /* plain FP64 state, geometry and operators     */
+#define MXM_PREC_FP32_MFMA 1 /* FP32 state and fluxes, volume + LIFT on FP32
+                                MFMA with FP32 accumulation; geometry,
+                                Jacobians, RK combine, positivity rescue and
+                                limiter means in FP64                        */
 
 /* ---- mesh descriptors -------------------------------------------------- */
 /* bc order [-x, +x, -y, +y, -z, +z], src/boundary.mojo:40-56 */
@@ -183,6 +187,47 @@ int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
  * it is the measurement hook that the roofline figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
 int mxm_dg_destroy(void *solver);
+
+/* ---- batch engine ----------------------------------------------------------
+ * No counterpart in the reference, which advances one problem per Solver /
+ * driver loop (the ensemble and serving drivers of examples/ run their
+ * problems one after another, e.g. examples/mhd_alfven_glm_2d_gpu.mojo:
+ * 215-242).  Many independent problems ("instances", each with its own
+ * mesh size, physics parameters, dt and step count) advance together: the
+ * instances that share a stage kernel are concatenated and one launch per
+ * RK stage steps all of them; one SSPRK3 step is a replayed HIP graph.
+ * Every instance follows mxm_dg_step_ssprk3 exactly, and its results do
+ * not depend on which other instances are in the batch.  The precision of
+ * the batch applies to every instance (MXM_PREC_*). */
+int mxm_batch_create(int precision, void **batch);
+/* Queue an instance: mesh, order, physics (id + prm[MXM_PRM_MAX]), time
+ * step dt, step count, limiter on/off with its eps, initial state q0
+ * [K*NP*NC] (reference order, FP64).  It joins at the next
+ * mxm_batch_step.  *id names it from then on. */
+int mxm_batch_add_2d(void *batch, const mxm_mesh2d_desc *desc, int P, int phys, const double *prm, double dt,
+                     int nsteps, int limiter, double eps, const double *q0, long long n, int *id);
+int mxm_batch_add_3d(void *batch, const mxm_mesh3d_desc *desc, int P, int phys, const double *prm, double dt,
+                     int nsteps, int limiter, double eps, const double *q0, long long n, int *id);
+/* Exact device bytes the instance would add to a batch (admission by HBM
+ * budget); host only. */
+int mxm_batch_footprint_2d(const mxm_mesh2d_desc *desc, int P, int phys, int precision, int limiter,
+                           long long *bytes);
+int mxm_batch_footprint_3d(const mxm_mesh3d_desc *desc, int P, int phys, int precision, int limiter,
+                           long long *bytes);
+/* nsteps SSPRK3 steps of every unfinished instance (asynchronous). */
+int mxm_batch_step(void *batch, int nsteps);
+/* Synchronises; ids[0..*count) = instances that finished all their steps
+ * since the last poll (at most cap). */
+int mxm_batch_poll(void *batch, int *ids, int cap, int *count);
+int mxm_batch_status(void *batch, int id, int *steps_done);
+/* State of one instance, reference order [K*NP*NC], FP64. */
+int mxm_batch_get_state(void *batch, int id, double *q, long long n);
+/* Drop an instance; its device memory is reclaimed at the next step. */
+int mxm_batch_remove(void *batch, int id);
+/* Device bytes held by the batch. */
+int mxm_batch_memory(void *batch, long long *bytes);
+int mxm_batch_sync(void *batch);
+int mxm_batch_destroy(void *batch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,585 @@
+// batch.hip -- the batch engine: many independent DG problems advanced
+// together, one stage launch per kernel group and RK stage.
+//
+// An instance is one problem (mesh, order, physics parameters, time step,
+// step count, initial state).  Instances that use the same stage kernel
+// (dimension, P, physics, flux variant, limiter) form a group whose tiles
+// are concatenated in one device arena: the stage kernel's block reads its
+// tile's instance and from it the physics parameters and time step
+// (StageArgs::tile_inst / inst_*), so one launch advances every instance
+// of the group by one stage.  A device step counter per instance turns the
+// instance off (inst_active = 0) after its last step, so instances with
+// different step counts share the launches and finished ones cost a block
+// exit.  One SSPRK3 step of a group (three stages, limiter, counter bump)
+// is captured in a HIP graph and replayed.
+//
+// Adding or removing instances marks the group dirty; the next step
+// repacks it: a new arena in the new member order, the placed states
+// copied device to device, new states uploaded, the tables rebuilt.
+// Results do not depend on the mix: a tile never mixes instances and every
+// element's arithmetic only reads its own instance.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mxm_dg.h"
+#include "limiter.hpp"
+#include "runtime.hpp"
+
+namespace mxm {
+namespace {
+
+__global__ void bump_steps_kernel(int *steps, const int *nsteps, int *active, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && active[i]) {
+    const int s = steps[i] + 1;
+    steps[i] = s;
+    if (s >= nsteps[i]) active[i] = 0;
+  }
+}
+
+struct BInst {
+  int id = 0, dim = 0, P = 0, phys = 0, variant = 0, NC = 0, K = 0, NP = 0;
+  bool limiter = false;
+  double eps = 0.1;
+  double prm[MXM_PRM_MAX] = {0};
+  double dt = 0.0;
+  int nsteps = 0, steps_done = 0;
+  bool placed = false, reported = false;
+  int base_tile = 0;
+  std::vector<double> q0;  // initial state, reference order, until placed
+  ElemTables t;
+};
+
+using GroupKey = std::tuple<int, int, int, int, int, long long>;  // dim, P, phys, variant, limiter, eps bits
+
+struct BGroup {
+  int dim = 0, P = 0, phys = 0, NC = 0, NP = 0, NFP = 0, NF = 0, epb = 0, prec = 0;
+  bool limiter = false;
+  double eps = 0.1;
+  StageLaunch launch = nullptr;
+  std::vector<int> members;  // instance ids in arena order
+  std::vector<int> packed;   // the member list the device arena holds
+  bool dirty = true;
+  int ntiles = 0, maxj = 0;
+  DevBuf<double> qd[3];
+  DevBuf<float> qf[3];
+  DevBuf<double> mean, invJ, inv_vol, jgeo, Dt, Lt, w, iprm, idt, ipost;
+  DevBuf<float> Aop;
+  DevBuf<int4> conn, jobs;
+  DevBuf<unsigned short> ipr;
+  DevBuf<int> fmap, tile_count, tile_inst, iactive, isteps, insteps;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  ~BGroup() { drop_graph(); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  size_t bytes() const {
+    size_t b = 0;
+    for (int k = 0; k < 3; ++k) b += qd[k].bytes() + qf[k].bytes();
+    return b + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + Dt.bytes() + Lt.bytes() + w.bytes() +
+           iprm.bytes() + idt.bytes() + ipost.bytes() + Aop.bytes() + conn.bytes() + jobs.bytes() + ipr.bytes() +
+           fmap.bytes() + tile_count.bytes() + tile_inst.bytes() + iactive.bytes() + isteps.bytes() +
+           insteps.bytes();
+  }
+};
+
+// Device bytes one instance adds to its group's arena (exact: the same
+// tables the repack builds).
+size_t instance_bytes(const ElemTables &t, int NP, int NC, int NFP, int dim, int prec, bool limiter) {
+  const size_t slots = (size_t)t.til.slots(), tiles = (size_t)t.til.ntiles, js = tiles * t.fj.maxj;
+  const size_t word = prec == MXM_PREC_FP64 ? sizeof(double) : sizeof(float);
+  size_t b = 3 * slots * NP * NC * word;                            // state trio
+  b += slots * (dim * dim + 1) * sizeof(double);                    // invJ, inv_vol
+  b += slots * (dim + 1) * sizeof(int4);                            // limiter neighbours
+  b += js * (sizeof(int4) + 4 * sizeof(double) + NFP * sizeof(unsigned short));  // jobs, geometry, partner nodes
+  b += tiles * 2 * sizeof(int);                                     // tile_count, tile_inst
+  b += (MXM_PRM_MAX + 2) * sizeof(double) + 3 * sizeof(int);        // per-instance scalars
+  if (limiter) b += slots * NC * sizeof(double);                    // cell means
+  return b;
+}
+
+struct Batch {
+  int prec = MXM_PREC_FP64;
+  hipStream_t stream = nullptr;
+  std::map<GroupKey, std::unique_ptr<BGroup>> groups;
+  std::map<int, std::unique_ptr<BInst>> insts;
+  std::map<std::pair<int, int>, RefElem> refs;
+  int next_id = 0;
+
+  ~Batch() {
+    groups.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  const RefElem &ref(int dim, int P) {
+    auto k = std::make_pair(dim, P);
+    auto it = refs.find(k);
+    if (it == refs.end()) it = refs.emplace(k, build_refelem(dim, P)).first;
+    return it->second;
+  }
+
+  BGroup &group_of(const BInst &in) {
+    long long eb = 0;
+    std::memcpy(&eb, &in.eps, sizeof(eb));
+    GroupKey key{in.dim, in.P, in.phys, in.variant, in.limiter ? 1 : 0, in.limiter ? eb : 0};
+    auto it = groups.find(key);
+    if (it != groups.end()) return *it->second;
+    auto g = std::make_unique<BGroup>();
+    const RefElem &re = ref(in.dim, in.P);
+    g->dim = in.dim, g->P = in.P, g->phys = in.phys, g->NC = in.NC, g->NP = re.NP, g->NFP = re.NFP, g->NF = re.NF;
+    g->epb = stage_epb(in.dim, re.NP);
+    g->prec = prec;
+    g->limiter = in.limiter, g->eps = in.eps;
+    g->launch = find_stage(in.dim, in.phys, in.variant, in.P, prec);
+    if (!g->launch) throw ArgError("no stage kernel for this (physics, flux, P)");
+    OpTables o = build_operators(re, in.dim, in.NC, prec);
+    if (prec == MXM_PREC_FP64) {
+      g->Dt.upload(o.Dt.data(), o.Dt.size());
+      g->Lt.upload(o.Lt.data(), o.Lt.size());
+    } else {
+      g->Aop.upload(o.Aop.data(), o.Aop.size());
+    }
+    g->w.upload(re.w.data(), re.w.size());
+    g->fmap.upload(re.fmap.data(), re.fmap.size());
+    return *groups.emplace(key, std::move(g)).first->second;
+  }
+
+  int add(int dim, const void *desc, int P, int phys, const double *prm, double dt, int nsteps, int limiter,
+          double eps, const double *q0, long long n) {
+    if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
+    const int nc = phys_nc(phys);
+    if (nc < 0) throw ArgError("unknown physics id");
+    if (dim == 3 && phys > MXM_PHYS_EULER) throw ArgError("2D module physics on a 3D mesh");
+    if (!(dt > 0.0) || nsteps < 0) throw ArgError("bad time step or step count");
+    if (limiter && nc > kLimMaxNC) throw ArgError("limiter supports NC <= 8");
+    auto in = std::make_unique<BInst>();
+    in->dim = dim, in->P = P, in->phys = phys, in->NC = nc;
+    if (prm) std::memcpy(in->prm, prm, sizeof(in->prm));
+    in->variant = phys_variant(phys, in->prm);
+    in->dt = dt, in->nsteps = nsteps, in->limiter = limiter != 0, in->eps = eps;
+    const RefElem &re = ref(dim, P);
+    HostMesh m;
+    if (dim == 3) {
+      auto d = static_cast<const mxm_mesh3d_desc *>(desc);
+      check_bc(d->bc, 6);
+      m = build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re);
+    } else {
+      auto d = static_cast<const mxm_mesh2d_desc *>(desc);
+      check_bc(d->bc, 4);
+      m = build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re);
+    }
+    in->K = m.K, in->NP = m.NP;
+    if (!q0 || n != (long long)m.K * m.NP * nc) throw ArgError("initial state length must be K*NP*NC");
+    in->q0.assign(q0, q0 + n);
+    in->t = build_elem_tables(m, re);
+    in->id = next_id++;
+    BGroup &g = group_of(*in);
+    g.members.push_back(in->id);
+    g.dirty = true;
+    const int id = in->id;
+    insts.emplace(id, std::move(in));
+    return id;
+  }
+
+  BInst &inst(int id) {
+    auto it = insts.find(id);
+    if (it == insts.end()) throw ArgError("unknown instance id");
+    return *it->second;
+  }
+
+  BGroup &group_with(int id) {
+    for (auto &kv : groups)
+      for (int m : kv.second->members)
+        if (m == id) return *kv.second;
+    throw ArgError("instance has no group");
+  }
+
+  // host mirrors of the device step counters of the packed arena
+  void pull_counters(BGroup &g) {
+    if (g.packed.empty() || !g.isteps.p) return;
+    std::vector<int> st(g.packed.size());
+    MXM_HIP(hipMemcpy(st.data(), g.isteps.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < g.packed.size(); ++k) {
+      auto it = insts.find(g.packed[k]);
+      if (it != insts.end() && it->second->placed) it->second->steps_done = st[k];
+    }
+  }
+
+  template <class R>
+  void place_states(BGroup &g, DevBuf<R> (&q)[3], const std::vector<int> &old_base) {
+    const size_t tile_vals = (size_t)g.epb * g.NP * g.NC;
+    DevBuf<R> nq[3];
+    for (auto &b : nq) {
+      b.alloc(std::max<size_t>(1, (size_t)g.ntiles * tile_vals));
+      MXM_HIP(hipMemset(b.p, 0, b.bytes()));
+    }
+    for (size_t k = 0; k < g.members.size(); ++k) {
+      BInst &in = inst(g.members[k]);
+      R *dst = nq[0].p + (size_t)in.base_tile * tile_vals;
+      const size_t nv = (size_t)in.t.til.ntiles * tile_vals;
+      if (in.placed) {
+        MXM_HIP(hipMemcpy(dst, q[0].p + (size_t)old_base[k] * tile_vals, nv * sizeof(R), hipMemcpyDeviceToDevice));
+      } else {
+        const size_t node = (size_t)g.NP * g.NC;
+        std::vector<R> d(nv, R(0));
+        for (int e = 0; e < in.K; ++e)
+          for (size_t c = 0; c < node; ++c) d[(size_t)in.t.til.dev_of[e] * node + c] = R(in.q0[(size_t)e * node + c]);
+        MXM_HIP(hipMemcpy(dst, d.data(), nv * sizeof(R), hipMemcpyHostToDevice));
+        in.placed = true;
+        in.q0.clear();
+        in.q0.shrink_to_fit();
+      }
+    }
+    for (int k = 0; k < 3; ++k) q[k].swap(nq[k]);
+  }
+
+  void repack(BGroup &g) {
+    MXM_HIP(hipStreamSynchronize(stream));
+    pull_counters(g);
+    g.drop_graph();
+    std::vector<int> old_base(g.members.size());
+    int tiles = 0, maxj = 1;
+    for (size_t k = 0; k < g.members.size(); ++k) {
+      BInst &in = inst(g.members[k]);
+      old_base[k] = in.base_tile;
+      in.base_tile = tiles;
+      tiles += in.t.til.ntiles;
+      maxj = std::max(maxj, in.t.fj.maxj);
+    }
+    g.ntiles = tiles;
+    g.maxj = maxj;
+    if (g.prec == MXM_PREC_FP64)
+      place_states(g, g.qd, old_base);
+    else
+      place_states(g, g.qf, old_base);
+    // concatenated tables, slots offset by each instance's base
+    const int epb = g.epb, dd = g.dim * g.dim, NF = g.NF, NFP = g.NFP;
+    const size_t slots = (size_t)tiles * epb;
+    std::vector<double> ij(slots * dd, 0.0), iv(slots, 0.0), geo((size_t)tiles * maxj * 4, 0.0);
+    std::vector<int4> cn(slots * NF), jb((size_t)tiles * maxj);
+    std::vector<unsigned short> pr((size_t)tiles * maxj * NFP, 0);
+    std::vector<int> tc(tiles, 0), ti(tiles, 0);
+    const size_t ni = g.members.size();
+    std::vector<double> iprm(ni * MXM_PRM_MAX), idt(ni), ipost(ni);
+    std::vector<int> iact(ni), ist(ni), inst_n(ni);
+    for (size_t k = 0; k < ni; ++k) {
+      BInst &in = inst(g.members[k]);
+      const ElemTables &t = in.t;
+      const int bt = in.base_tile, bs = bt * epb, lt = t.til.ntiles, lmj = t.fj.maxj;
+      for (int s = 0; s < lt * epb; ++s) {
+        for (int q = 0; q < dd; ++q) ij[(size_t)(bs + s) * dd + q] = t.invJ[(size_t)s * dd + q];
+        iv[bs + s] = t.inv_vol[s];
+        for (int lf = 0; lf < NF; ++lf) {
+          int4 c = t.conn[(size_t)s * NF + lf];
+          c.x += bs;
+          cn[(size_t)(bs + s) * NF + lf] = c;
+        }
+      }
+      for (int T = 0; T < lt; ++T) {
+        tc[bt + T] = t.til.count[T];
+        ti[bt + T] = (int)k;
+        for (int j = 0; j < maxj; ++j) {
+          const size_t dj = (size_t)(bt + T) * maxj + j;
+          if (j < lmj) {
+            const size_t sj = (size_t)T * lmj + j;
+            int4 r{t.fj.jobs[sj * 4], t.fj.jobs[sj * 4 + 1] + bs, t.fj.jobs[sj * 4 + 2], t.fj.jobs[sj * 4 + 3]};
+            jb[dj] = r;
+            for (int c = 0; c < 4; ++c) geo[dj * 4 + c] = t.fj.geo[sj * 4 + c];
+            for (int r2 = 0; r2 < NFP; ++r2) pr[dj * NFP + r2] = t.fj.ipr[sj * NFP + r2];
+          } else {
+            jb[dj] = int4{-1, (bt + T) * epb, 0, 0};
+          }
+        }
+      }
+      std::memcpy(&iprm[k * MXM_PRM_MAX], in.prm, sizeof(in.prm));
+      idt[k] = in.dt;
+      ipost[k] = post_factor(in.phys, in.prm, in.dt);
+      ist[k] = in.steps_done;
+      inst_n[k] = in.nsteps;
+      iact[k] = in.steps_done < in.nsteps ? 1 : 0;
+    }
+    g.invJ.upload(ij.data(), ij.size());
+    g.inv_vol.upload(iv.data(), iv.size());
+    g.conn.upload(cn.data(), cn.size());
+    g.jobs.upload(jb.data(), jb.size());
+    g.jgeo.upload(geo.data(), geo.size());
+    g.ipr.upload(pr.data(), pr.size());
+    g.tile_count.upload(tc.data(), tc.size());
+    g.tile_inst.upload(ti.data(), ti.size());
+    g.iprm.upload(iprm.data(), iprm.size());
+    g.idt.upload(idt.data(), idt.size());
+    g.ipost.upload(ipost.data(), ipost.size());
+    g.iactive.upload(iact.data(), iact.size());
+    g.isteps.upload(ist.data(), ist.size());
+    g.insteps.upload(inst_n.data(), inst_n.size());
+    if (g.limiter) g.mean.alloc(slots * g.NC);
+    g.packed = g.members;
+    g.dirty = false;
+    if (g.ntiles > 0) capture(g);
+  }
+
+  template <class R>
+  void launch_step(BGroup &g, DevBuf<R> (&q)[3]) {
+    const int ni = (int)g.members.size();
+    for (int s = 0; s < 3; ++s) {
+      StageArgs<R> a{};
+      const StagePlan pl = stage_plan(s);
+      a.q_in = q[pl.in].p, a.q_a = q[pl.qa].p, a.q_out = q[pl.out].p;
+      a.a = pl.a, a.b = pl.b, a.c = pl.c;
+      a.tile_inst = g.tile_inst.p, a.inst_prm = g.iprm.p, a.inst_dt = g.idt.p, a.inst_post = g.ipost.p;
+      a.inst_active = g.iactive.p;
+      a.dt = 0.0, a.post_scale = 1.0;
+      // GLM decay per instance (factor 1 where alpha_d = 0)
+      a.post_comp = s == 2 && g.phys == MXM_PHYS_MHDGLM2D ? 6 : -1;
+      a.tile_count = g.tile_count.p;
+      a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p;
+      a.Dt = g.Dt.p, a.Lt = g.Lt.p, a.Aop = g.Aop.p, a.fmap = g.fmap.p;
+      a.jobs = g.jobs.p, a.jgeo = g.jgeo.p, a.ipr = g.ipr.p, a.maxj = g.maxj, a.ntiles = g.ntiles;
+      MXM_HIP(g.launch(&a, stream));
+      if (g.limiter)
+        MXM_HIP(launch_limiter<R>(a.q_out, g.mean.p, g.w.p, g.conn.p, g.ntiles * g.epb, g.NP, g.NC, g.NF, g.dim,
+                                  g.eps, stream, g.tile_inst.p, g.iactive.p, g.epb));
+    }
+    hipLaunchKernelGGL(bump_steps_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, g.isteps.p, g.insteps.p,
+                       g.iactive.p, ni);
+    MXM_HIP(hipGetLastError());
+  }
+
+  void launch_step(BGroup &g) {
+    if (g.prec == MXM_PREC_FP64)
+      launch_step(g, g.qd);
+    else
+      launch_step(g, g.qf);
+  }
+
+  // one SSPRK3 step of the group as a HIP graph (kernel nodes only)
+  void capture(BGroup &g) {
+    MXM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_step(g);
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    MXM_HIP(hipStreamEndCapture(stream, &g.graph));
+    MXM_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  }
+
+  void step(int n) {
+    for (auto &kv : groups)
+      if (kv.second->dirty) repack(*kv.second);
+    for (int k = 0; k < n; ++k)
+      for (auto &kv : groups) {
+        BGroup &g = *kv.second;
+        if (g.ntiles == 0) continue;
+        MXM_HIP(hipGraphLaunch(g.exec, stream));
+      }
+  }
+
+  void sync_counters() {
+    MXM_HIP(hipStreamSynchronize(stream));
+    for (auto &kv : groups) pull_counters(*kv.second);
+  }
+
+  template <class R>
+  void read_state(BGroup &g, DevBuf<R> (&q)[3], BInst &in, double *h) {
+    const size_t node = (size_t)g.NP * g.NC, tile_vals = (size_t)g.epb * node;
+    const size_t nv = (size_t)in.t.til.ntiles * tile_vals;
+    std::vector<R> d(nv);
+    MXM_HIP(hipMemcpy(d.data(), q[0].p + (size_t)in.base_tile * tile_vals, nv * sizeof(R), hipMemcpyDeviceToHost));
+    for (int e = 0; e < in.K; ++e)
+      for (size_t c = 0; c < node; ++c) h[(size_t)e * node + c] = double(d[(size_t)in.t.til.dev_of[e] * node + c]);
+  }
+
+  void get_state(int id, double *h, long long n) {
+    BInst &in = inst(id);
+    if (!h || n != (long long)in.K * in.NP * in.NC) throw ArgError("state length must be K*NP*NC");
+    if (!in.placed) {
+      std::memcpy(h, in.q0.data(), n * sizeof(double));
+      return;
+    }
+    BGroup &g = group_with(id);
+    if (g.dirty) repack(g);
+    MXM_HIP(hipStreamSynchronize(stream));
+    if (g.prec == MXM_PREC_FP64)
+      read_state(g, g.qd, in, h);
+    else
+      read_state(g, g.qf, in, h);
+  }
+
+  void remove(int id) {
+    BInst &in = inst(id);
+    (void)in;
+    BGroup &g = group_with(id);
+    MXM_HIP(hipStreamSynchronize(stream));
+    pull_counters(g);
+    g.members.erase(std::find(g.members.begin(), g.members.end(), id));
+    g.dirty = true;
+    insts.erase(id);
+  }
+
+  size_t bytes() const {
+    size_t b = 0;
+    for (auto &kv : groups) b += kv.second->bytes();
+    return b;
+  }
+};
+
+Batch *as_batch(void *h) {
+  if (!h) throw ArgError("null batch handle");
+  return static_cast<Batch *>(h);
+}
+
+int footprint(int dim, const void *desc, int P, int phys, int prec, int limiter, long long *out) {
+  if (!desc || !out) throw ArgError("null argument");
+  if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
+  const int nc = phys_nc(phys);
+  if (nc < 0) throw ArgError("unknown physics id");
+  RefElem re = build_refelem(dim, P);
+  HostMesh m;
+  if (dim == 3) {
+    auto d = static_cast<const mxm_mesh3d_desc *>(desc);
+    check_bc(d->bc, 6);
+    m = build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re);
+  } else {
+    auto d = static_cast<const mxm_mesh2d_desc *>(desc);
+    check_bc(d->bc, 4);
+    m = build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re);
+  }
+  ElemTables t = build_elem_tables(m, re);
+  *out = (long long)instance_bytes(t, m.NP, nc, m.NFP, dim, prec, limiter != 0);
+  return MXM_OK;
+}
+
+}  // namespace
+}  // namespace mxm
+
+using namespace mxm;
+
+extern "C" {
+
+int mxm_batch_create(int precision, void **out) {
+  return guarded([&] {
+    if (!out) throw ArgError("null out handle");
+    *out = nullptr;
+    if (g_device < 0) return fail(MXM_ERR_NOGPU, "mxm_hip_init has not been called");
+    if (precision != MXM_PREC_FP64 && precision != MXM_PREC_FP32_MFMA) throw ArgError("unknown precision");
+    auto b = std::make_unique<Batch>();
+    b->prec = precision;
+    MXM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b.release();
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_add_2d(void *h, const mxm_mesh2d_desc *desc, int P, int phys, const double *prm, double dt, int nsteps,
+                     int limiter, double eps, const double *q0, long long n, int *id) {
+  return guarded([&] {
+    if (!desc || !id) throw ArgError("null argument");
+    *id = as_batch(h)->add(2, desc, P, phys, prm, dt, nsteps, limiter, eps, q0, n);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_add_3d(void *h, const mxm_mesh3d_desc *desc, int P, int phys, const double *prm, double dt, int nsteps,
+                     int limiter, double eps, const double *q0, long long n, int *id) {
+  return guarded([&] {
+    if (!desc || !id) throw ArgError("null argument");
+    *id = as_batch(h)->add(3, desc, P, phys, prm, dt, nsteps, limiter, eps, q0, n);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_footprint_2d(const mxm_mesh2d_desc *desc, int P, int phys, int precision, int limiter,
+                           long long *bytes) {
+  return guarded([&] { return footprint(2, desc, P, phys, precision, limiter, bytes); });
+}
+
+int mxm_batch_footprint_3d(const mxm_mesh3d_desc *desc, int P, int phys, int precision, int limiter,
+                           long long *bytes) {
+  return guarded([&] { return footprint(3, desc, P, phys, precision, limiter, bytes); });
+}
+
+int mxm_batch_step(void *h, int nsteps) {
+  return guarded([&] {
+    if (nsteps < 0) throw ArgError("negative step count");
+    as_batch(h)->step(nsteps);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_poll(void *h, int *ids, int cap, int *count) {
+  return guarded([&] {
+    Batch *b = as_batch(h);
+    if (!count || (cap > 0 && !ids)) throw ArgError("null argument");
+    b->sync_counters();
+    int k = 0;
+    for (auto &kv : b->insts) {
+      BInst &in = *kv.second;
+      if (!in.reported && in.steps_done >= in.nsteps && k < cap) {
+        ids[k++] = in.id;
+        in.reported = true;
+      }
+    }
+    *count = k;
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_status(void *h, int id, int *steps_done) {
+  return guarded([&] {
+    Batch *b = as_batch(h);
+    if (!steps_done) throw ArgError("null argument");
+    b->sync_counters();
+    *steps_done = b->inst(id).steps_done;
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_get_state(void *h, int id, double *q, long long n) {
+  return guarded([&] {
+    as_batch(h)->get_state(id, q, n);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_remove(void *h, int id) {
+  return guarded([&] {
+    as_batch(h)->remove(id);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_memory(void *h, long long *bytes) {
+  return guarded([&] {
+    if (!bytes) throw ArgError("null argument");
+    *bytes = (long long)as_batch(h)->bytes();
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_sync(void *h) {
+  return guarded([&] {
+    MXM_HIP(hipStreamSynchronize(as_batch(h)->stream));
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_destroy(void *h) {
+  delete static_cast<Batch *>(h);
+  return MXM_OK;
+}
+
+}  // extern "C"

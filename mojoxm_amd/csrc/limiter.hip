@@ -18,28 +18,31 @@
 
 namespace mxm {
 
-__global__ __launch_bounds__(256) void cell_mean_kernel(const double *__restrict__ q,
+template <class R>
+__global__ __launch_bounds__(256) void cell_mean_kernel(const R *__restrict__ q,
                                                         const double *__restrict__ w, int K, int NP,
                                                         int NC, double *__restrict__ mean) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= K * NC) return;
   const int e = t / NC, c = t - e * NC;
-  const double *qe = q + (size_t)e * NP * NC + c;
+  const R *qe = q + (size_t)e * NP * NC + c;
   double s = 0.0;
-  for (int n = 0; n < NP; ++n) s += w[n] * qe[n * NC];
+  for (int n = 0; n < NP; ++n) s += w[n] * double(qe[n * NC]);
   mean[t] = s;
 }
 
-__global__ __launch_bounds__(256) void bj_theta_apply_kernel(double *__restrict__ q,
+template <class R>
+__global__ __launch_bounds__(256) void bj_theta_apply_kernel(R *__restrict__ q,
                                                              const double *__restrict__ mean,
                                                              const int4 *__restrict__ conn, int K,
                                                              int NP, int NC, int NF, int all_comps,
-                                                             double eps2) {
+                                                             double eps2, const int *__restrict__ tile_inst,
+                                                             const int *__restrict__ inst_active, int tepb) {
   __shared__ double sTheta[256];
   const int epb = 256 / NP;
   const int el = threadIdx.x / NP, n = threadIdx.x - el * NP;
   const int e = blockIdx.x * epb + el;
-  const bool active = el < epb && e < K;
+  const bool active = el < epb && e < K && !(inst_active && !inst_active[tile_inst[e / tepb]]);
   double theta = 1.0;
   double own[kLimMaxNC], mn[kLimMaxNC], mx[kLimMaxNC], qv[kLimMaxNC];
   const int ncl = all_comps ? NC : 1;
@@ -54,8 +57,8 @@ __global__ __launch_bounds__(256) void bj_theta_apply_kernel(double *__restrict_
         mx[c] = v > mx[c] ? v : mx[c];
       }
     }
-    const double *qn = q + ((size_t)e * NP + n) * NC;
-    for (int c = 0; c < NC; ++c) qv[c] = qn[c];
+    const R *qn = q + ((size_t)e * NP + n) * NC;
+    for (int c = 0; c < NC; ++c) qv[c] = double(qn[c]);
     for (int c = 0; c < ncl; ++c) {
       const double pD = mx[c] - own[c] < 0.0 ? 0.0 : mx[c] - own[c];
       const double nD = own[c] - mn[c] < 0.0 ? 0.0 : own[c] - mn[c];
@@ -77,22 +80,29 @@ __global__ __launch_bounds__(256) void bj_theta_apply_kernel(double *__restrict_
     th = v < th ? v : th;
   }
   if (!(th < 1.0)) return;
-  double *qn = q + ((size_t)e * NP + n) * NC;
+  R *qn = q + ((size_t)e * NP + n) * NC;
   for (int c = 0; c < NC; ++c) {
     const double bm = mean[(size_t)e * NC + c];
-    qn[c] = bm + th * (qv[c] - bm);
+    qn[c] = R(bm + th * (qv[c] - bm));
   }
 }
 
-hipError_t launch_limiter(double *q, double *mean, const double *w, const int4 *conn, int K, int NP,
-                          int NC, int NF, int dim, double eps, hipStream_t s) {
+template <class R>
+hipError_t launch_limiter(R *q, double *mean, const double *w, const int4 *conn, int K, int NP, int NC,
+                          int NF, int dim, double eps, hipStream_t s, const int *tile_inst,
+                          const int *inst_active, int epb_tiles) {
   if (K <= 0) return hipSuccess;
   if (NC > kLimMaxNC || NP > 256) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(cell_mean_kernel, dim3((K * NC + 255) / 256), dim3(256), 0, s, q, w, K, NP, NC, mean);
+  hipLaunchKernelGGL(cell_mean_kernel<R>, dim3((K * NC + 255) / 256), dim3(256), 0, s, q, w, K, NP, NC, mean);
   const int epb = 256 / NP;
-  hipLaunchKernelGGL(bj_theta_apply_kernel, dim3((K + epb - 1) / epb), dim3(256), 0, s, q, mean, conn, K, NP,
-                     NC, NF, dim == 3 ? 1 : 0, eps * eps);
+  hipLaunchKernelGGL(bj_theta_apply_kernel<R>, dim3((K + epb - 1) / epb), dim3(256), 0, s, q, mean, conn, K, NP,
+                     NC, NF, dim == 3 ? 1 : 0, eps * eps, tile_inst, inst_active, epb_tiles);
   return hipGetLastError();
 }
+
+template hipError_t launch_limiter<double>(double *, double *, const double *, const int4 *, int, int, int, int,
+                                           int, double, hipStream_t, const int *, const int *, int);
+template hipError_t launch_limiter<float>(float *, double *, const double *, const int4 *, int, int, int, int,
+                                          int, double, hipStream_t, const int *, const int *, int);
 
 }  // namespace mxm

@@ -28,6 +28,9 @@ template <class R>
 MXM_HD R rmin(R a, R b) { return a < b ? a : b; }
 template <class R>
 MXM_HD R rabs(R a) { return a >= R(0) ? a : -a; }
+// square root in the precision of the argument (host and device)
+MXM_HD float msqrt(float x) { return sqrtf(x); }
+MXM_HD double msqrt(double x) { return sqrt(x); }
 
 // ------------------------------------------------------------ Advection --
 // src/advection.mojo:60-119
@@ -37,7 +40,7 @@ struct Advection {
   static constexpr bool kSource = false, kLimit = false;
   R v[3];
   R inflow;
-  static Advection from(const double *p) { return {{R(p[0]), R(p[1]), R(p[2])}, R(p[3])}; }
+  MXM_HD static Advection from(const double *p) { return {{R(p[0]), R(p[1]), R(p[2])}, R(p[3])}; }
   template <int ND>
   MXM_HD void flux(const R *q, R *F) const {
     for (int d = 0; d < ND; ++d) F[d] = v[d] * q[0];
@@ -52,7 +55,8 @@ struct Advection {
   // (src/advection.mojo:98-112)
   MXM_HD void ghost(const R *, int bc, const R *, R *g) const { g[0] = bc == BC_INFLOW ? inflow : R(0); }
   MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
-  MXM_HD void limit(R *) const {}
+  template <class T>
+  MXM_HD void limit(T *) const {}
 };
 
 // ---------------------------------------------------------------- Euler --
@@ -66,11 +70,13 @@ struct Euler {
   R gamma, rmin_, pmin_;
   R g[3];
   R inflow[5];
-  static Euler from(const double *p) {
+  double gamma_d, rmin_d, pmin_d;  // FP64 copies for the positivity rescue
+  MXM_HD static Euler from(const double *p) {
     Euler e;
     e.gamma = R(p[0]);
     e.rmin_ = R(p[1]);
     e.pmin_ = R(p[2]);
+    e.gamma_d = p[0], e.rmin_d = p[1], e.pmin_d = p[2];
     for (int d = 0; d < 3; ++d) e.g[d] = R(p[5 + d]);
     for (int c = 0; c < 5; ++c) e.inflow[c] = R(p[8 + c]);
     return e;
@@ -135,7 +141,7 @@ struct Euler {
     ax -= n[0] * pr;
     ay -= n[1] * pr;
     az -= n[2] * pr;
-    const R imag = R(1) / sqrt(ax * ax + ay * ay + az * az);
+    const R imag = R(1) / msqrt(ax * ax + ay * ay + az * az);
     for (int d = 0; d < 3; ++d) F.n[d] = n[d];
     F.t[0] = ax * imag;
     F.t[1] = ay * imag;
@@ -158,20 +164,20 @@ struct Euler {
     for (int d = 0; d < 3; ++d) q[1 + d] = F.n[d] * r[1] + F.t[d] * r[2] + F.b[d] * r[3];
   }
   // Roe averages, src/euler.mojo:161-176 (returns u, v, w, H, a):
-  // m/sqrt(rho) = sqrt(rho) u, so every term shares one reciprocal.
+  // m/msqrt(rho) = msqrt(rho) u, so every term shares one reciprocal.
   MXM_HD void roe_avg(const R *ql, const R *qr, const Prim &L, const Prim &Rt, R *o) const {
-    const R sl = sqrt(L.r), sr = sqrt(Rt.r), is = R(1) / (sl + sr);
+    const R sl = msqrt(L.r), sr = msqrt(Rt.r), is = R(1) / (sl + sr);
     for (int d = 0; d < 3; ++d) o[d] = (sl * L.u[d] + sr * Rt.u[d]) * is;
     o[3] = (sl * ((ql[4] + L.p) * L.ir) + sr * ((qr[4] + Rt.p) * Rt.ir)) * is;
     R u2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
-    o[4] = sqrt(rmax((gamma - R(1)) * (o[3] - R(0.5) * u2), R(0)));
+    o[4] = msqrt(rmax((gamma - R(1)) * (o[3] - R(0.5) * u2), R(0)));
   }
   // Harten-Hyman entropy fix, src/euler.mojo:207-288.  wave[m*3 + w].
   MXM_HD void entropy_fixed(const R *ql, const R *qr, const R *s, const R *wave, R *amdq,
                             R *apdq) const {
     const R gm = gamma;
     R rl = rho(ql), pl = pres(ql);
-    R s0 = ql[1] / rl - sqrt(gm * pl / rl);
+    R s0 = ql[1] / rl - msqrt(gm * pl / rl);
     for (int m = 0; m < 5; ++m) amdq[m] = R(0);
     if (!(s0 > R(0) && s[0] > R(0))) {
       R a[5];
@@ -179,19 +185,19 @@ struct Euler {
       a[0] = rmax(a[0], rmin_);
       R ke = R(0.5) * (a[1] * a[1] + a[2] * a[2] + a[3] * a[3]) / a[0];
       R p1 = rmax((gm - R(1)) * (a[4] - ke), pmin_);
-      R s1 = a[1] / a[0] - sqrt(gm * p1 / a[0]);
+      R s1 = a[1] / a[0] - msqrt(gm * p1 / a[0]);
       R sf = (s0 < R(0) && s1 > R(0)) ? s0 * (s1 - s[0]) / (s1 - s0) : (s[0] < R(0) ? s[0] : R(0));
       for (int m = 0; m < 5; ++m) amdq[m] = sf * wave[m * 3];
       if (s[1] < R(0)) {
         for (int m = 0; m < 5; ++m) amdq[m] += s[1] * wave[m * 3 + 1];
         R rr = rho(qr), prr = pres(qr);
-        R s3 = qr[1] / rr + sqrt(gm * prr / rr);
+        R s3 = qr[1] / rr + msqrt(gm * prr / rr);
         R b[5];
         for (int m = 0; m < 5; ++m) b[m] = qr[m] - wave[m * 3 + 2];
         b[0] = rmax(b[0], rmin_);
         R ke2 = R(0.5) * (b[1] * b[1] + b[2] * b[2] + b[3] * b[3]) / b[0];
         R p2 = rmax((gm - R(1)) * (b[4] - ke2), pmin_);
-        R s2 = b[1] / b[0] + sqrt(gm * p2 / b[0]);
+        R s2 = b[1] / b[0] + msqrt(gm * p2 / b[0]);
         R sf2 = (s2 < R(0) && s3 > R(0)) ? s2 * (s3 - s[2]) / (s3 - s2) : (s[2] < R(0) ? s[2] : R(0));
         for (int m = 0; m < 5; ++m) amdq[m] += sf2 * wave[m * 3 + 2];
       }
@@ -207,7 +213,7 @@ struct Euler {
     const Prim L = prim(rl), Rt = prim(rr);
     const R dl = L.r, dr = Rt.r, pl = L.p, pr = Rt.p, ul = L.u[0], ur = Rt.u[0];
     if constexpr (FLUX == 0) {
-      R cl = sqrt(gamma * pl * L.ir), cr = sqrt(gamma * pr * Rt.ir);
+      R cl = msqrt(gamma * pl * L.ir), cr = msqrt(gamma * pr * Rt.ir);
       R cm = rmax(rabs(ul) + cl, rabs(ur) + cr);
       rf[0] = R(0.5) * (rl[1] + rr[1] + cm * (dl - dr));
       rf[1] = R(0.5) * (rl[1] * ul + pl + rr[1] * ur + pr + cm * (rl[1] - rr[1]));
@@ -239,8 +245,8 @@ struct Euler {
         wave[14] = a5 * (H + u * a);
         s[0] = u - a, s[1] = u, s[2] = u + a;
       } else {
-        s[0] = rmin(ul - sqrt(gamma * pl * L.ir), ra[0] - ra[4]);
-        s[2] = rmax(ur + sqrt(gamma * pr * Rt.ir), ra[0] + ra[4]);
+        s[0] = rmin(ul - msqrt(gamma * pl * L.ir), ra[0] - ra[4]);
+        s[2] = rmax(ur + msqrt(gamma * pr * Rt.ir), ra[0] + ra[4]);
         if constexpr (FLUX == 2) {
           // HLLE, src/euler.mojo:364-412
           s[1] = R(0);
@@ -333,23 +339,25 @@ struct Euler {
     s[4] = q[1] * g[0] + q[2] * g[1] + q[3] * g[2];
   }
   // NaN-safe positivity rescue, src/euler.mojo:758-792
-  MXM_HD void limit(R *q) const {
-    const R e0 = pmin_ / (gamma - R(1));
-    if (!(q[0] > rmin_)) {
-      q[0] = rmin_;
-      q[1] = q[2] = q[3] = R(0);
+  // Evaluated in FP64 whatever the state precision (the stage kernels pass
+  // an FP64 copy of the combined node).
+  MXM_HD void limit(double *q) const {
+    const double e0 = pmin_d / (gamma_d - 1.0);
+    if (!(q[0] > rmin_d)) {
+      q[0] = rmin_d;
+      q[1] = q[2] = q[3] = 0.0;
       q[4] = e0;
       return;
     }
-    R m2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
-    if (!(m2 >= R(0))) {
-      q[1] = q[2] = q[3] = R(0);
+    double m2 = q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    if (!(m2 >= 0.0)) {
+      q[1] = q[2] = q[3] = 0.0;
       q[4] = e0;
       return;
     }
-    R ke = R(0.5) * m2 / q[0];
-    R p = (gamma - R(1)) * (q[4] - ke);
-    if (!(p > pmin_)) q[4] = ke + e0;
+    double ke = 0.5 * m2 / q[0];
+    double p = (gamma_d - 1.0) * (q[4] - ke);
+    if (!(p > pmin_d)) q[4] = ke + e0;
   }
 };
 
@@ -366,7 +374,7 @@ struct Euler2D {
   static constexpr bool kSource = true, kLimit = false;
   R gamma, rmin_, pmin_, gx, gy;
   R inflow[4];
-  static Euler2D from(const double *p) {
+  MXM_HD static Euler2D from(const double *p) {
     Euler2D e;
     e.gamma = R(p[0]), e.rmin_ = R(p[1]), e.pmin_ = R(p[2]), e.gx = R(p[4]), e.gy = R(p[5]);
     for (int c = 0; c < 4; ++c) e.inflow[c] = R(p[6 + c]);
@@ -398,14 +406,14 @@ struct Euler2D {
       R FL[8], FR[8];
       flux<2>(ql, FL);
       flux<2>(qr, FR);
-      R sl = sqrt(ul * ul + vl * vl) + sqrt(gamma * pl / rl);
-      R sr = sqrt(ur * ur + vr * vr) + sqrt(gamma * pr / rr);
+      R sl = msqrt(ul * ul + vl * vl) + msqrt(gamma * pl / rl);
+      R sr = msqrt(ur * ur + vr * vr) + msqrt(gamma * pr / rr);
       R al = sl > sr ? sl : sr;
       for (int c = 0; c < 4; ++c)
         f[c] = R(0.5) * ((FL[c] + FR[c]) * nx + (FL[4 + c] + FR[4 + c]) * ny) - R(0.5) * al * (qr[c] - ql[c]);
     } else {
       // src/local_mesh_2d_gpu_euler.mojo:466-545
-      R al = sqrt(gamma * pl / rl), ar = sqrt(gamma * pr / rr);
+      R al = msqrt(gamma * pl / rl), ar = msqrt(gamma * pr / rr);
       R unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
       R FnL[4] = {ql[1] * nx + ql[2] * ny, (ql[1] * ul + pl) * nx + (ql[1] * vl) * ny,
                   (ql[1] * vl) * nx + (ql[2] * vl + pl) * ny, (ul * (ql[3] + pl)) * nx + (vl * (ql[3] + pl)) * ny};
@@ -446,7 +454,8 @@ struct Euler2D {
     R r = q[0] < rmin_ ? rmin_ : q[0];
     s[0] = R(0), s[1] = r * gx, s[2] = r * gy, s[3] = q[1] * gx + q[2] * gy;
   }
-  MXM_HD void limit(R *) const {}
+  template <class T>
+  MXM_HD void limit(T *) const {}
 };
 
 // src/local_mesh_2d_gpu_sw.mojo; HLL when HLL == 1
@@ -456,7 +465,7 @@ struct SW2D {
   static constexpr bool kSource = false, kLimit = false;
   R grav, hmin;
   R inflow[3];
-  static SW2D from(const double *p) { return {R(p[0]), R(p[1]), {R(p[3]), R(p[4]), R(p[5])}}; }
+  MXM_HD static SW2D from(const double *p) { return {R(p[0]), R(p[1]), {R(p[3]), R(p[4]), R(p[5])}}; }
   template <int ND>
   MXM_HD void flux(const R *q, R *F) const {
     R h = q[0] < hmin ? hmin : q[0];
@@ -473,12 +482,12 @@ struct SW2D {
     R hl = ql[0] < hmin ? hmin : ql[0], hr = qr[0] < hmin ? hmin : qr[0];
     R ul = ql[1] / hl, vl = ql[2] / hl, ur = qr[1] / hr, vr = qr[2] / hr;
     if constexpr (HLL == 0) {
-      R sl = sqrt(ul * ul + vl * vl) + sqrt(grav * hl), sr = sqrt(ur * ur + vr * vr) + sqrt(grav * hr);
+      R sl = msqrt(ul * ul + vl * vl) + msqrt(grav * hl), sr = msqrt(ur * ur + vr * vr) + msqrt(grav * hr);
       R al = sl > sr ? sl : sr;
       for (int c = 0; c < 3; ++c)
         f[c] = R(0.5) * ((FL[c] + FR[c]) * nx + (FL[3 + c] + FR[3 + c]) * ny) - R(0.5) * al * (qr[c] - ql[c]);
     } else {
-      R cl = sqrt(grav * hl), cr = sqrt(grav * hr);
+      R cl = msqrt(grav * hl), cr = msqrt(grav * hr);
       R unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
       R SL = rmin(unl - cl, unr - cr), SR = rmax(unl + cl, unr + cr);
       R FnL[3], FnR[3];
@@ -504,7 +513,8 @@ struct SW2D {
     g[2] = in ? inflow[2] : (wall ? q[2] - R(2) * mn * n[1] : q[2]);
   }
   MXM_HD void source(const R *, R *s) const { s[0] = s[1] = s[2] = R(0); }
-  MXM_HD void limit(R *) const {}
+  template <class T>
+  MXM_HD void limit(T *) const {}
 };
 
 // src/local_mesh_2d_gpu_mhd_glm.mojo (GLM-MHD, Rusanov)
@@ -514,7 +524,7 @@ struct MHDGLM2D {
   static constexpr bool kSource = false, kLimit = false;
   R gamma, rmin_, pmin_, ch;
   R inflow[7];
-  static MHDGLM2D from(const double *p) {
+  MXM_HD static MHDGLM2D from(const double *p) {
     MHDGLM2D e;
     e.gamma = R(p[0]), e.rmin_ = R(p[1]), e.pmin_ = R(p[2]), e.ch = R(p[3]);
     for (int c = 0; c < 7; ++c) e.inflow[c] = R(p[5 + c]);
@@ -540,7 +550,7 @@ struct MHDGLM2D {
     R cs2 = gamma * p / r, ca2 = BB / r, s = cs2 + ca2;
     R disc = s * s - R(4) * cs2 * (Bx * Bx) / r;
     if (disc < R(0)) disc = R(0);
-    speed = rmax(sqrt(u * u + v * v) + sqrt(R(0.5) * (s + sqrt(disc))), ch);
+    speed = rmax(msqrt(u * u + v * v) + msqrt(R(0.5) * (s + msqrt(disc))), ch);
   }
   template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
@@ -566,7 +576,8 @@ struct MHDGLM2D {
   MXM_HD void source(const R *, R *s) const {
     for (int c = 0; c < 7; ++c) s[c] = R(0);
   }
-  MXM_HD void limit(R *) const {}
+  template <class T>
+  MXM_HD void limit(T *) const {}
 };
 
 // src/local_mesh_2d_gpu_maxwell.mojo (Rusanov with alpha = c, PEC walls)
@@ -575,7 +586,7 @@ struct Maxwell2D {
   static constexpr int NC = 6;
   static constexpr bool kSource = true, kLimit = false;
   R c, J[3], M[3], inflow[6];
-  static Maxwell2D from(const double *p) {
+  MXM_HD static Maxwell2D from(const double *p) {
     Maxwell2D m;
     m.c = R(p[0]);
     for (int d = 0; d < 3; ++d) m.J[d] = R(p[1 + d]), m.M[d] = R(p[4 + d]);
@@ -616,7 +627,8 @@ struct Maxwell2D {
     R c2 = c * c;
     for (int d = 0; d < 3; ++d) s[d] = -(c2 * J[d]), s[3 + d] = -M[d];
   }
-  MXM_HD void limit(R *) const {}
+  template <class T>
+  MXM_HD void limit(T *) const {}
 };
 
 // src/local_mesh_2d_gpu_advection.mojo (scalar upwind)
@@ -625,7 +637,7 @@ struct Advection2D {
   static constexpr int NC = 1;
   static constexpr bool kSource = false, kLimit = false;
   R vx, vy, inflow;
-  static Advection2D from(const double *p) { return {R(p[0]), R(p[1]), R(p[2])}; }
+  MXM_HD static Advection2D from(const double *p) { return {R(p[0]), R(p[1]), R(p[2])}; }
   template <int ND>
   MXM_HD void flux(const R *q, R *F) const {
     F[0] = vx * q[0];
@@ -640,7 +652,8 @@ struct Advection2D {
   // (src/local_mesh_2d_gpu_advection.mojo:285-298)
   MXM_HD void ghost(const R *, int bc, const R *, R *g) const { g[0] = bc == BC_INFLOW ? inflow : R(0); }
   MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
-  MXM_HD void limit(R *) const {}
+  template <class T>
+  MXM_HD void limit(T *) const {}
 };
 
 }  // namespace mxm

@@ -15,77 +15,26 @@
 #include "../../include/mxm_dg.h"
 #include "host.hpp"
 #include "limiter.hpp"
-#include "stage.hpp"
+#include "runtime.hpp"
 
 namespace mxm {
 namespace {
 
-thread_local std::string g_err;
-int g_device = -1;
-
-int fail(int code, const std::string &msg) {
-  g_err = msg;
-  return code;
-}
-
-#define MXM_HIP(call)                                                                          \
-  do {                                                                                         \
-    hipError_t e_ = (call);                                                                    \
-    if (e_ != hipSuccess) throw HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct HipError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-struct ArgError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-template <class T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf &) = delete;
-  DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  void alloc(size_t count) {
-    n = count;
-    MXM_HIP(hipMalloc(&p, (count ? count : 1) * sizeof(T)));
-  }
-  void upload(const T *h, size_t count) {
-    alloc(count);
-    if (count) MXM_HIP(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
-  }
-  size_t bytes() const { return n * sizeof(T); }
-};
-
-int phys_nc(int phys) {
-  static const int nc[7] = {1, 5, 4, 3, 7, 6, 1};
-  return (phys >= 0 && phys < 7) ? nc[phys] : -1;
-}
-
-int phys_variant(int phys, const double *prm) {
-  // Euler: flux kind + 4 * entropy fix (the fix only acts on the wave solvers)
-  if (phys == MXM_PHYS_EULER) return (int)prm[3] + (prm[3] != 0.0 && prm[4] != 0.0 ? 4 : 0);
-  if (phys == MXM_PHYS_EULER2D) return prm[3] != 0.0 ? 1 : 0;
-  if (phys == MXM_PHYS_SW2D) return prm[2] != 0.0 ? 1 : 0;
-  return 0;
-}
-
 struct Solver {
   int dim = 0, P = 0, NP = 0, NFP = 0, NF = 0, NC = 0, phys = 0;
-  int K = 0, Nf = 0, nperm = 0;
+  int K = 0, Nf = 0;
   double prm[MXM_PRM_MAX] = {0};
+  int prec = MXM_PREC_FP64;
   StageLaunch launch = nullptr;
   hipStream_t stream = nullptr;
   bool limiter = false;
   double eps = 0.1;
-  DevBuf<double> q, q1, q2, mean;
-  DevBuf<double> invJ, inv_vol, jgeo, Dt, Lt, w;
+  // state trio q, q1, q2: FP64 (MXM_PREC_FP64) or FP32 (MXM_PREC_FP32_MFMA)
+  DevBuf<double> q, q1, q2;
+  DevBuf<float> f, f1, f2;
+  DevBuf<double> mean;
+  DevBuf<double> invJ, inv_vol, jgeo, Dt, Lt, w, dprm;
+  DevBuf<float> Aop;
   DevBuf<int4> conn;
   DevBuf<int> fmap, tile_count;
   DevBuf<int4> jobs;
@@ -103,88 +52,70 @@ struct Solver {
 
   void build(const HostMesh &m, const RefElem &re) {
     dim = m.dim, P = m.P, NP = m.NP, NFP = m.NFP, NF = m.NF, K = m.K, Nf = m.Nf;
-    Connectivity cn = build_connectivity(m, re);
-    nperm = cn.nperm;
-    til = build_tiling(m, stage_epb(dim, NP), NFP);
+    ElemTables t = build_elem_tables(m, re);
+    til = std::move(t.til);
     Kd = til.slots();
     ntiles = til.ntiles;
     MXM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     const size_t nq = (size_t)Kd * NP * NC;
-    q.alloc(nq);
-    q1.alloc(nq);
-    q2.alloc(nq);
-    MXM_HIP(hipMemset(q.p, 0, q.bytes()));
-    MXM_HIP(hipMemset(q1.p, 0, q1.bytes()));
-    MXM_HIP(hipMemset(q2.p, 0, q2.bytes()));
-    mean.alloc((size_t)Kd * NC);
-    // per-element tables in device order; a padding slot is its own
-    // neighbour with a zero state, which the limiter leaves unchanged
-    const int dd = dim * dim;
-    std::vector<double> ij((size_t)Kd * dd, 0.0), iv((size_t)Kd, 0.0);
-    std::vector<int4> cd((size_t)Kd * NF);
-    for (int sl = 0; sl < Kd; ++sl) {
-      const int e = til.ref_of[sl];
-      for (int lf = 0; lf < NF; ++lf) {
-        const int32_t *c = &cn.conn[((size_t)(e < 0 ? 0 : e) * NF + lf) * 4];
-        cd[(size_t)sl * NF + lf] = e < 0 ? int4{sl, 0, 0, 0} : int4{til.dev_of[c[0]], c[1], c[2], 0};
+    if (prec == MXM_PREC_FP64) {
+      for (DevBuf<double> *b : {&q, &q1, &q2}) {
+        b->alloc(nq);
+        MXM_HIP(hipMemset(b->p, 0, b->bytes()));
       }
-      if (e < 0) continue;
-      for (int k = 0; k < dd; ++k) ij[(size_t)sl * dd + k] = m.elem_invJ[(size_t)e * dd + k];
-      iv[sl] = m.elem_inv_vol[e];
+    } else {
+      for (DevBuf<float> *b : {&f, &f1, &f2}) {
+        b->alloc(nq);
+        MXM_HIP(hipMemset(b->p, 0, b->bytes()));
+      }
     }
-    invJ.upload(ij.data(), ij.size());
-    inv_vol.upload(iv.data(), iv.size());
-    conn.upload(cd.data(), cd.size());
-    FaceJobs fj = build_face_jobs(m, re, cn, til);
-    maxj = fj.maxj;
+    mean.alloc((size_t)Kd * NC);
+    invJ.upload(t.invJ.data(), t.invJ.size());
+    inv_vol.upload(t.inv_vol.data(), t.inv_vol.size());
+    conn.upload(t.conn.data(), t.conn.size());
+    maxj = t.fj.maxj;
     tile_count.upload(til.count.data(), til.count.size());
-    jobs.upload(reinterpret_cast<const int4 *>(fj.jobs.data()), fj.jobs.size() / 4);
-    jgeo.upload(fj.geo.data(), fj.geo.size());
-    ipr.upload(fj.ipr.data(), fj.ipr.size());
-    // transposed operators, node columns zero-padded as the kernel expects
-    const int npad = stage_node_pad(dim, NP, NC);
-    std::vector<double> dt((size_t)dim * NP * npad, 0.0), lt((size_t)NF * NFP * npad, 0.0);
-    for (int d = 0; d < dim; ++d)
-      for (int i = 0; i < NP; ++i)
-        for (int j = 0; j < NP; ++j) dt[((size_t)d * NP + j) * npad + i] = re.D[((size_t)d * NP + i) * NP + j];
-    for (int f = 0; f < NF; ++f)
-      for (int i = 0; i < NP; ++i)
-        for (int r = 0; r < NFP; ++r) lt[((size_t)f * NFP + r) * npad + i] = re.Lift[((size_t)f * NP + i) * NFP + r];
-    Dt.upload(dt.data(), dt.size());
-    Lt.upload(lt.data(), lt.size());
+    jobs.upload(reinterpret_cast<const int4 *>(t.fj.jobs.data()), t.fj.jobs.size() / 4);
+    jgeo.upload(t.fj.geo.data(), t.fj.geo.size());
+    ipr.upload(t.fj.ipr.data(), t.fj.ipr.size());
+    OpTables o = build_operators(re, dim, NC, prec);
+    if (prec == MXM_PREC_FP64) {
+      Dt.upload(o.Dt.data(), o.Dt.size());
+      Lt.upload(o.Lt.data(), o.Lt.size());
+    } else {
+      Aop.upload(o.Aop.data(), o.Aop.size());
+    }
     w.upload(re.w.data(), re.w.size());
     fmap.upload(re.fmap.data(), re.fmap.size());
+    dprm.upload(prm, MXM_PRM_MAX);
     MXM_HIP(hipDeviceSynchronize());
   }
 
   // Shu-Osher plan of src/ssprk3.mojo:41-78 (FP64 coefficients).
   void stage(int s, double dt) {
-    StageArgs<double> a{};
-    // (q_in, q_a, q_b, q_out): (q, q, q, q1), (q1, q, q1, q2), (q2, q, q2, q);
-    // q_b is always the stage input, which the kernel reads from its tile
-    if (s == 0) {
-      a.q_in = q.p, a.q_a = q.p, a.q_out = q1.p, a.a = 1.0, a.b = 0.0, a.cdt = 1.0 * dt;
-    } else if (s == 1) {
-      a.q_in = q1.p, a.q_a = q.p, a.q_out = q2.p, a.a = 0.75, a.b = 0.25, a.cdt = 0.25 * dt;
-    } else {
-      a.q_in = q2.p, a.q_a = q.p, a.q_out = q.p, a.a = 1.0 / 3.0, a.b = 2.0 / 3.0, a.cdt = (2.0 / 3.0) * dt;
-    }
+    if (prec == MXM_PREC_FP64)
+      stage_with(s, dt, q.p, q1.p, q2.p);
+    else
+      stage_with(s, dt, f.p, f1.p, f2.p);
+  }
+
+  template <class R>
+  void stage_with(int s, double dt, R *b0, R *b1, R *b2) {
+    StageArgs<R> a{};
+    R *trio[3] = {b0, b1, b2};
+    const StagePlan pl = stage_plan(s);
+    a.q_in = trio[pl.in], a.q_a = trio[pl.qa], a.q_out = trio[pl.out];
+    a.a = pl.a, a.b = pl.b, a.c = pl.c;
+    a.inst_prm = dprm.p;  // one instance: tile_inst, inst_dt, inst_post, inst_active stay null
+    a.dt = dt;
     a.tile_count = tile_count.p;
     a.invJ = invJ.p, a.inv_vol = inv_vol.p;
-    a.Dt = Dt.p, a.Lt = Lt.p, a.fmap = fmap.p;
+    a.Dt = Dt.p, a.Lt = Lt.p, a.Aop = Aop.p, a.fmap = fmap.p;
     a.jobs = jobs.p, a.jgeo = jgeo.p, a.ipr = ipr.p, a.maxj = maxj, a.ntiles = ntiles;
-    a.post_comp = -1;
-    a.post_scale = 1.0;
-    // GLM psi damping q[psi] *= exp(-alpha_d dt) once per step after the
-    // last stage (src/local_mesh_2d_gpu_mhd_glm.mojo:464-484), fused into
-    // the stage-3 epilogue; the limiter is linear per component with theta
-    // from component 0, so the two commute.
-    if (s == 2 && phys == MXM_PHYS_MHDGLM2D && prm[4] > 0.0) {
-      a.post_comp = 6;
-      a.post_scale = std::exp(-prm[4] * dt);
-    }
-    MXM_HIP(launch(a, prm, stream));
-    if (limiter) MXM_HIP(launch_limiter(a.q_out, mean.p, w.p, conn.p, Kd, NP, NC, NF, dim, eps, stream));
+    a.post_comp = s == 2 ? post_component(phys, prm) : -1;
+    a.post_scale = post_factor(phys, prm, dt);
+    MXM_HIP(launch(&a, stream));
+    if (limiter) MXM_HIP(launch_limiter<R>(a.q_out, mean.p, w.p, conn.p, Kd, NP, NC, NF, dim, eps, stream));
   }
 
   void step(double dt, int n) {
@@ -193,33 +124,39 @@ struct Solver {
   }
 
   long long dof() const { return (long long)K * NP * NC; }
-  // reference order [K][NP][NC] <-> device slots
-  void to_device(const double *h, double *d) const {
-    const size_t node = (size_t)NP * NC;
-    std::fill(d, d + (size_t)Kd * node, 0.0);
-    for (int e = 0; e < K; ++e) std::memcpy(d + (size_t)til.dev_of[e] * node, h + (size_t)e * node, node * sizeof(double));
+  size_t state_elem_bytes() const { return prec == MXM_PREC_FP64 ? sizeof(double) : sizeof(float); }
+  // q (reference order [K][NP][NC], FP64) <-> device slots and precision
+  void upload_state(const double *h) {
+    const size_t node = (size_t)NP * NC, n = (size_t)Kd * node;
+    MXM_HIP(hipStreamSynchronize(stream));
+    if (prec == MXM_PREC_FP64) {
+      std::vector<double> d(n, 0.0);
+      for (int e = 0; e < K; ++e) std::memcpy(&d[(size_t)til.dev_of[e] * node], h + (size_t)e * node, node * sizeof(double));
+      MXM_HIP(hipMemcpy(q.p, d.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> d(n, 0.f);
+      for (int e = 0; e < K; ++e)
+        for (size_t k = 0; k < node; ++k) d[(size_t)til.dev_of[e] * node + k] = float(h[(size_t)e * node + k]);
+      MXM_HIP(hipMemcpy(f.p, d.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
   }
-  void to_host(const double *d, double *h) const {
-    const size_t node = (size_t)NP * NC;
-    for (int e = 0; e < K; ++e) std::memcpy(h + (size_t)e * node, d + (size_t)til.dev_of[e] * node, node * sizeof(double));
+  void download_state(double *h) {
+    const size_t node = (size_t)NP * NC, n = (size_t)Kd * node;
+    MXM_HIP(hipStreamSynchronize(stream));
+    if (prec == MXM_PREC_FP64) {
+      std::vector<double> d(n);
+      MXM_HIP(hipMemcpy(d.data(), q.p, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int e = 0; e < K; ++e) std::memcpy(h + (size_t)e * node, &d[(size_t)til.dev_of[e] * node], node * sizeof(double));
+    } else {
+      std::vector<float> d(n);
+      MXM_HIP(hipMemcpy(d.data(), f.p, n * sizeof(float), hipMemcpyDeviceToHost));
+      for (int e = 0; e < K; ++e)
+        for (size_t k = 0; k < node; ++k) h[(size_t)e * node + k] = double(d[(size_t)til.dev_of[e] * node + k]);
+    }
   }
-  long long state_bytes_per_step() const { return 8LL * dof() * (long long)sizeof(double); }
+  size_t state_bytes() const { return q.bytes() + q1.bytes() + q2.bytes() + f.bytes() + f1.bytes() + f2.bytes(); }
+  long long state_bytes_per_step() const { return 8LL * dof() * (long long)state_elem_bytes(); }
 };
-
-template <class F>
-int guarded(F &&f) {
-  try {
-    return f();
-  } catch (const HipError &e) {
-    return fail(MXM_ERR_HIP, e.what());
-  } catch (const ArgError &e) {
-    return fail(MXM_ERR_ARG, e.what());
-  } catch (const std::bad_alloc &) {
-    return fail(MXM_ERR_ALLOC, "host allocation failed");
-  } catch (const std::exception &e) {
-    return fail(MXM_ERR_MESH, e.what());
-  }
-}
 
 Solver *as_solver(void *h) {
   if (!h) throw ArgError("null solver handle");
@@ -231,16 +168,11 @@ HostMesh *as_mesh(void *h) {
   return static_cast<HostMesh *>(h);
 }
 
-void check_bc(const int *bc, int n) {
-  for (int k = 0; k < n; ++k)
-    if (bc[k] < 0 || bc[k] > 3) throw ArgError("boundary kind out of range");
-}
-
 int create(int dim, const void *desc, int P, int phys, const double *prm_in, int precision, void **out) {
   if (!out) throw ArgError("null out handle");
   *out = nullptr;
   if (g_device < 0) return fail(MXM_ERR_NOGPU, "mxm_hip_init has not been called");
-  if (precision != MXM_PREC_FP64) throw ArgError("only MXM_PREC_FP64 is available");
+  if (precision != MXM_PREC_FP64 && precision != MXM_PREC_FP32_MFMA) throw ArgError("unknown precision");
   if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
   const int nc = phys_nc(phys);
   if (nc < 0) throw ArgError("unknown physics id");
@@ -248,14 +180,10 @@ int create(int dim, const void *desc, int P, int phys, const double *prm_in, int
   auto s = std::make_unique<Solver>();
   s->phys = phys;
   s->NC = nc;
+  s->prec = precision;
   if (prm_in) std::memcpy(s->prm, prm_in, sizeof(s->prm));
   const int variant = phys_variant(phys, s->prm);
-  if (dim == 3)
-    s->launch = find_stage_3d(phys, variant, P);
-  else if (phys <= MXM_PHYS_EULER)
-    s->launch = find_stage_2d_trait(phys, variant, P);
-  else
-    s->launch = find_stage_2d_module(phys, variant, P);
+  s->launch = find_stage(dim, phys, variant, P, precision);
   if (!s->launch) throw ArgError("no stage kernel for this (physics, flux, P)");
   RefElem re = build_refelem(dim, P);
   if (dim == 3) {
@@ -301,11 +229,15 @@ int mxm_get_constant(int which, int *value) {
     int n = 0;
     for (int P = 1; P <= 5; ++P) {
       // Euler variants: flux kind + 4 * entropy fix
-      for (int v = 0; v < 8; ++v) n += (find_stage_3d(1, v, P) != nullptr) + (find_stage_2d_trait(1, v, P) != nullptr);
-      n += (find_stage_3d(0, 0, P) != nullptr) + (find_stage_2d_trait(0, 0, P) != nullptr);
+      for (int v = 0; v < 8; ++v)
+        n += (stage_3d_fp64(1, v, P) != nullptr) + (stage_2d_trait_fp64(1, v, P) != nullptr) +
+             (stage_3d_fp32(1, v, P) != nullptr) + (stage_2d_trait_fp32(1, v, P) != nullptr);
+      n += (stage_3d_fp64(0, 0, P) != nullptr) + (stage_2d_trait_fp64(0, 0, P) != nullptr) +
+           (stage_3d_fp32(0, 0, P) != nullptr) + (stage_2d_trait_fp32(0, 0, P) != nullptr);
       for (int ph = 2; ph <= 6; ++ph)
         for (int v = 0; v < 2; ++v)
-          if ((ph == 2 || ph == 3) || v == 0) n += find_stage_2d_module(ph, v, P) != nullptr;
+          if ((ph == 2 || ph == 3) || v == 0)
+            n += (stage_2d_module_fp64(ph, v, P) != nullptr) + (stage_2d_module_fp32(ph, v, P) != nullptr);
     }
     *value = n;
     return MXM_OK;
@@ -434,10 +366,7 @@ int mxm_dg_upload_q(void *h, const double *qh, long long n) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!qh || n != s->dof()) throw ArgError("state length must be K*NP*NC");
-    std::vector<double> dev(s->q.n);
-    s->to_device(qh, dev.data());
-    MXM_HIP(hipStreamSynchronize(s->stream));
-    MXM_HIP(hipMemcpy(s->q.p, dev.data(), s->q.bytes(), hipMemcpyHostToDevice));
+    s->upload_state(qh);
     return MXM_OK;
   });
 }
@@ -446,10 +375,7 @@ int mxm_dg_download_q(void *h, double *qh, long long n) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!qh || n != s->dof()) throw ArgError("state length must be K*NP*NC");
-    std::vector<double> dev(s->q.n);
-    MXM_HIP(hipStreamSynchronize(s->stream));
-    MXM_HIP(hipMemcpy(dev.data(), s->q.p, s->q.bytes(), hipMemcpyDeviceToHost));
-    s->to_host(dev.data(), qh);
+    s->download_state(qh);
     return MXM_OK;
   });
 }
@@ -458,10 +384,8 @@ int mxm_dg_download_component(void *h, int c, double *out, long long n) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!out || c < 0 || c >= s->NC || n != (long long)s->K * s->NP) throw ArgError("bad component download");
-    std::vector<double> dev(s->q.n), all((size_t)s->dof());
-    MXM_HIP(hipStreamSynchronize(s->stream));
-    MXM_HIP(hipMemcpy(dev.data(), s->q.p, s->q.bytes(), hipMemcpyDeviceToHost));
-    s->to_host(dev.data(), all.data());
+    std::vector<double> all((size_t)s->dof());
+    s->download_state(all.data());
     for (long long k = 0; k < n; ++k) out[k] = all[(size_t)k * s->NC + c];
     return MXM_OK;
   });
@@ -544,8 +468,8 @@ int mxm_dg_memory_report(void *h, mxm_memory_report *o) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!o) throw ArgError("null out");
-    o->rk_stage = (long long)(s->q.bytes() + s->q1.bytes() + s->q2.bytes());
-    o->dg_operators = (long long)(s->Dt.bytes() + s->Lt.bytes() + s->w.bytes() + s->fmap.bytes());
+    o->rk_stage = (long long)s->state_bytes();
+    o->dg_operators = (long long)(s->Dt.bytes() + s->Lt.bytes() + s->Aop.bytes() + s->w.bytes() + s->fmap.bytes());
     o->limiter = (long long)s->mean.bytes();
     o->mesh_connectivity = (long long)(s->invJ.bytes() + s->inv_vol.bytes() + s->conn.bytes() +
                                        s->jgeo.bytes() + s->jobs.bytes() + s->ipr.bytes() +

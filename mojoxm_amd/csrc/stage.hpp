@@ -9,29 +9,38 @@
 // (for 2D) the face-flux + vol_lift_combine_rk kernel pair of
 // src/local_mesh_2d_gpu_<phys>.mojo fused into one pass.
 //
-// Work decomposition (256 threads = 4 wave64 per block, EPB elements per
-// block):
-//   1. each thread starts the loads of its first face item (job record,
-//      face normal and area, partner node, and the partner's trace when the
-//      partner lies outside the tile); meanwhile the block stages its
-//      contiguous q_in tile, the tile's geometry and (when small) the
-//      reference operators in LDS with coalesced loads;
-//   2. per (element, node): the nodal flux, mapped to reference directions
-//      by invJ, into LDS;
-//      per face item: the numerical flux (boundary faces: against the
+// One 256-thread block (4 wave64) per tile of at most EPB elements, which
+// are contiguous in device memory:
+//   1. each thread starts the loads of its first face item (job record
+//      with the face geometry, partner node, and the partner's trace when
+//      the partner lies outside the tile); the block stages the q_in tile
+//      and the tile's geometry in LDS;
+//   2. per face item: the numerical flux (boundary faces: against the
 //      physics' ghost state), signed and scaled by the face area, into LDS.
 //      A face whose two elements are both in the tile is evaluated once by
 //      its side-0 element, which also stores the negated value in the
 //      partner's slot; faces to elements outside the tile are evaluated by
 //      each side, as the reference evaluates every face from both sides;
-//      side 0 is always the left state;
-//   3. per (element, node): the D and Lift contractions over LDS, source,
-//      RK combine, limit_state, store.
+//      side 0 is always the left state.  Per (element, node): the nodal
+//      flux mapped to reference directions by invJ, into LDS;
+//   3. the D and Lift contractions over LDS, then per (element, node) the
+//      RK combine, limit_state and the store.
+//
+// Two arithmetic forms share the kernel:
+//   R = double  plain FP64: the contractions run on the VALU in FP64;
+//   R = float   FP32-MFMA: FP32 state, fluxes and traces; the volume and
+//               LIFT contractions are one [D | Lift] x [F ; S] product per
+//               tile on V_MFMA_F32_16X16X4F32 with FP32 accumulation, the
+//               surface fluxes pre-scaled by -inv_vol s area; geometry and
+//               Jacobians stay FP64 (the nodal flux is mapped by invJ in
+//               FP64), and so do the RK combine and the positivity rescue.
 // Blocks are mapped to tiles so that each of the 8 XCDs walks a contiguous
 // range of tiles: face neighbours then mostly hit the XCD's own L2.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "physics.hpp"
 
@@ -43,22 +52,35 @@ template <class R>
 struct StageArgs {
   const R *q_in, *q_a;
   R *q_out;
-  R a, b, cdt;
+  double a, b, c;         // stage coefficients; c dt multiplies the rhs
+  // Instances: a tile belongs to one instance, whose physics parameters
+  // (MXM_PRM_MAX doubles), time step and GLM decay the block reads.  One
+  // solver is a single instance (tile_inst == nullptr: instance 0 with dt
+  // and post_scale below); a batch sets the per-instance tables, and an
+  // instance whose inst_active entry is 0 is skipped.
+  const int *tile_inst;     // [ntiles] or nullptr
+  const double *inst_prm;   // [n_inst][MXM_PRM_MAX]
+  const double *inst_dt;    // [n_inst] or nullptr -> dt
+  const double *inst_post;  // [n_inst] or nullptr -> post_scale
+  const int *inst_active;   // [n_inst] or nullptr (all active)
+  double dt, post_scale;
+  int post_comp;            // component scaled after the combine (GLM psi), -1 none
   const int *tile_count;  // [ntiles] elements in each tile (slots T*EPB ..)
-  const R *invJ;       // [slots][ND*ND] rows = reference directions
-  const R *inv_vol;    // [slots] 1/(6V) or 1/(2A)
-  const R *Dt;         // [ND*NP (k,j)][NPAD (i)] transposed D_ref, zero padded
-  const R *Lt;         // [NF*NFP (f,r)][NPAD (i)] transposed Lift_ref, zero padded
-  const int *fmap;     // [NF][NFP] element node of each reference face slot
-  const int4 *jobs;    // [ntiles*maxj] {el | lf<<7 | pin<<9 | plf<<16 | pslot<<18
-                       //  (-1: padding), partner slot, face id, side | bc<<1}
-  const double *jgeo;  // [ntiles*maxj][4] face normal, area
+  const double *invJ;     // [slots][ND*ND] rows = reference directions
+  const double *inv_vol;  // [slots] 1/(6V) or 1/(2A)
+  const double *Dt;       // FP64 form: [ND*NP (k,j)][NPAD (i)] transposed D_ref, zero padded
+  const double *Lt;       // FP64 form: [NF*NFP (f,r)][NPAD (i)] transposed Lift_ref, zero padded
+  const float *Aop;       // FP32 form: [16*MT (i)][KPAD (k)] = [D_0 .. D_ND-1 | Lift], zero padded
+  const int *fmap;        // [NF][NFP] element node of each reference face slot
+  const int4 *jobs;       // [ntiles*maxj] {el | lf<<7 | pin<<9 | plf<<16 | pslot<<18
+                          //  (-1: padding), partner slot, face id, side | bc<<1}
+  const double *jgeo;     // [ntiles*maxj][4] face normal, area
   const unsigned short *ipr;  // [ntiles*maxj*NFP] partner node | partner slot << 8
-  int maxj;            // job slots per tile
+  int maxj;               // job slots per tile
   int ntiles;
-  int post_comp;       // component scaled after the combine (GLM psi), -1 none
-  R post_scale;
 };
+
+constexpr int kPrmMax = 16;  // MXM_PRM_MAX
 
 // Elements per 256-thread block for NP nodes per element, rounded down to
 // whole lattice cells (2 triangles, 6 tets) when a cell fits: tiles are
@@ -68,7 +90,7 @@ constexpr int stage_epb(int ND, int NP) {
   return 256 / NP >= (ND == 3 ? 6 : 2) ? 256 / NP - (256 / NP) % (ND == 3 ? 6 : 2) : 256 / NP;
 }
 
-// Contraction layout: the tile's (element, component) columns span
+// FP64 contraction layout: the tile's (element, component) columns span
 // ceil(EPB*NC/64) waves; the block's 4 waves form `groups` groups, each
 // producing `IH` consecutive output nodes, so the operator entries a wave
 // reads are wave-uniform.  The transposed operators are stored with
@@ -79,6 +101,11 @@ constexpr int stage_groups(int ND, int NP, int NC) {
 constexpr int stage_node_pad(int ND, int NP, int NC) {
   return stage_groups(ND, NP, NC) * ((NP + stage_groups(ND, NP, NC) - 1) / stage_groups(ND, NP, NC));
 }
+
+// FP32-MFMA operator shape: 16-row tiles of output nodes, K = ND*NP volume
+// columns then NF*NFP lift columns, padded to the MFMA depth of 4.
+constexpr int mfma_mtiles(int NP) { return (NP + 15) / 16; }
+constexpr int mfma_kpad(int ND, int NP, int NFP) { return (ND * NP + (ND + 1) * NFP + 3) / 4 * 4; }
 
 template <int ND, int P>
 struct Shape {
@@ -102,7 +129,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
 template <class R, int NC>
 struct FaceItem {
   int packed, r, side, bc, pn, rs, pl;
-  R n[3], area;
+  R n[3];
+  double area;
   R qp[NC];  // partner trace when the partner is outside the tile
 };
 
@@ -123,7 +151,7 @@ __device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int tile, 
   it.side = jr.w & 1;
   it.bc = (jr.w >> 1) & 7;
   it.n[0] = R(g0.x), it.n[1] = R(g0.y), it.n[2] = R(g1.x);
-  it.area = R(g1.y);
+  it.area = g1.y;
   it.pn = pr & 255;
   it.rs = pr >> 8;
   it.pl = jr.y - e0;
@@ -135,34 +163,34 @@ __device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int tile, 
   }
 }
 
+typedef float mxm_float4 __attribute__((ext_vector_type(4)));
+
 template <class Phys, int ND, int P, class R>
-__global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys) {
+__global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A) {
   using S = Shape<ND, P>;
+  constexpr bool kMFMA = std::is_same<R, float>::value;
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
   constexpr int TILE = EPB * NP * NC;  // values of one element tile
-  // Contraction layout: the (element, component) columns of the tile are
-  // spread over one or more waves; the block's waves split into GROUPS
-  // groups that each produce a contiguous range of IH output nodes, so the
-  // operator entries a wave needs are wave-uniform (scalar loads).
-  constexpr int GROUPS = stage_groups(ND, NP, NC);
-  constexpr int GSIZE = S::BLOCK / GROUPS;
-  constexpr int NPAD = stage_node_pad(ND, NP, NC);
-  constexpr int IH = NPAD / GROUPS;
-  constexpr int ITER = (EPB * NC + GSIZE - 1) / GSIZE;  // columns per thread, at most
+  constexpr int NCOL = EPB * NC;       // (element, component) columns of a tile
   // sW holds ND tile-sized planes.  Plane 0 first holds the q_in tile as
   // [el][j][c]; after the face phase the planes hold the contravariant
   // nodal fluxes as [k][j][el][c]; after the contraction plane 0 holds
-  // c dt (volume - lift) as [el][i][c].
+  // (volume - lift) as [el][i][c].
   __shared__ R sW[ND * TILE];
-  __shared__ R sS[NF * NFP * EPB * NC];  // [f*NFP + r][el][c]
-  __shared__ R sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
+  __shared__ R sS[NF * NFP * NCOL];           // [f*NFP + r][el][c]
+  __shared__ double sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
 
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int inst = A.tile_inst ? A.tile_inst[tile] : 0;
+  if (A.inst_active && !A.inst_active[inst]) return;
   const int e0 = tile * EPB;
   const int nE = A.tile_count[tile];
   if (nE <= 0) return;
+  const Phys phys = Phys::from(A.inst_prm + (size_t)inst * kPrmMax);
+  const double cdt = A.c * (A.inst_dt ? A.inst_dt[inst] : A.dt);
+  const double post = A.inst_post ? A.inst_post[inst] : A.post_scale;
   const int nitems = A.maxj * NFP;  // padding items have packed < 0
   // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
   const bool node = tid < nE * NP;
@@ -174,7 +202,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
     const R *src = A.q_in + (size_t)e0 * NP * NC;
     const int n = nE * NP * NC;
     for (int t = tid; t < n; t += S::BLOCK) sW[t] = src[t];
-    const R *gJ = A.invJ + (size_t)e0 * ND * ND;
+    const double *gJ = A.invJ + (size_t)e0 * ND * ND;
     for (int t = tid; t < nE * ND * ND; t += S::BLOCK) sJ[t] = gJ[t];
     for (int t = tid; t < nE; t += S::BLOCK) sJ[EPB * ND * ND + t] = A.inv_vol[e0 + t];
     for (int t = tid; t < NF * NFP; t += S::BLOCK) sMap[t] = A.fmap[t];
@@ -209,44 +237,48 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
       for (int c = 0; c < NC; ++c) qr[c] = it.side == 0 ? qo[c] : qm[c];
     R Fs[NC];
     phys.template numerical_flux<ND>(ql, qr, it.n, Fs);
-    const R s = it.side == 0 ? it.area : -it.area;
+    const double sa = it.side == 0 ? it.area : -it.area;
+    // FP64 form: s area F*, lifted and scaled by -inv_vol in the
+    // contraction; FP32 form: -inv_vol s area F*, ready for the product
+    const R own = kMFMA ? R(-sJ[EPB * ND * ND + el] * sa) : R(sa);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sS[((lf * NFP + it.r) * EPB + el) * NC + c] = s * Fs[c];
+    for (int c = 0; c < NC; ++c) sS[(lf * NFP + it.r) * NCOL + el * NC + c] = own * Fs[c];
     if (pin) {
       const int plf = (it.packed >> 16) & 3, rs = it.rs;
+      const R oth = kMFMA ? R(sJ[EPB * ND * ND + pin - 1] * sa) : R(-sa);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sS[((plf * NFP + rs) * EPB + (pin - 1)) * NC + c] = -(s * Fs[c]);
+      for (int c = 0; c < NC; ++c) sS[(plf * NFP + rs) * NCOL + (pin - 1) * NC + c] = oth * Fs[c];
     }
   }
 
   // q_a of the own node: loaded only now, so its registers are not held
   // through the face phase (the register peak)
-  R qa[NC];
+  double qa[NC];
   if (node) {
     const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) qa[c] = g[c];
+    for (int c = 0; c < NC; ++c) qa[c] = double(g[c]);
   }
 
   // ---- 2b. nodal flux of the own node, and the RK terms of q_in ----------
-  // qa becomes a q_a + b q_in + c dt S(q_in); the flux, mapped to
-  // reference directions by invJ, waits in registers until every read of
-  // the q_in tile is done.
+  // qa becomes a q_a + b q_in + c dt S(q_in) (FP64); the flux, mapped to
+  // reference directions by invJ in FP64, waits in registers until every
+  // read of the q_in tile is done.
   R Fk[ND * NC];
   if (node) {
     const int el = tid / NP;
     const R *q = &sQ[tid * NC];
     R F[ND * NC];
     phys.template flux<ND>(q, F);
-    const R *iJ = &sJ[el * ND * ND];
+    const double *iJ = &sJ[el * ND * ND];
 #pragma unroll
     for (int k = 0; k < ND; ++k)
 #pragma unroll
       for (int c = 0; c < NC; ++c) {
-        R v = R(0);
+        double v = 0.0;
 #pragma unroll
-        for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * F[d * NC + c];
-        Fk[k * NC + c] = v;
+        for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * double(F[d * NC + c]);
+        Fk[k * NC + c] = R(v);
       }
     R src[NC];
 #pragma unroll
@@ -255,7 +287,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
       if (phys.source_active()) phys.source(q, src);
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) qa[c] = A.a * qa[c] + A.b * q[c] + A.cdt * src[c];
+    for (int c = 0; c < NC; ++c) qa[c] = A.a * qa[c] + A.b * double(q[c]) + cdt * double(src[c]);
   }
   __syncthreads();
   if (node) {
@@ -267,82 +299,142 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A, Phys phys
   }
   __syncthreads();
 
-  // ---- 3a. contractions per (element, component) column ------------------
-  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr
-  const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
-  const int i0 = grp * IH;
-  const int lane = tid - grp * GSIZE;
-  R acc[ITER][IH];
+  // ---- 3a. contractions ---------------------------------------------------
+  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr,
+  // written into plane 0 as [el][i][c] after a barrier
+  if constexpr (kMFMA) {
+    // Out[i][col] = sum_k Aop[i][k] In[k][col], In = [flux planes ; sS].
+    // V_MFMA_F32_16X16X4F32 operands, lane l = 16 g + r: A[r][k0 + g],
+    // B[k0 + g][col0 + r]; result rows 4 g + v (v < 4), column col0 + r.
+    constexpr int KV = ND * NP, K = KV + NF * NFP, KPAD = mfma_kpad(ND, NP, NFP);
+    constexpr int MT = mfma_mtiles(NP);
+    constexpr int NT = (NCOL + 15) / 16;  // 16-column tiles
+    constexpr int NTW = (NT + 3) / 4;     // tiles per wave
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int l = tid & 63, lr = l & 15, lg = l >> 4;
+    mxm_float4 acc[MT][NTW];
 #pragma unroll
-  for (int m = 0; m < ITER; ++m) {
-    const int p = lane + m * GSIZE;
+    for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int ii = 0; ii < IH; ++ii) acc[m][ii] = R(0);
-    if (p < nE * NC) {
-      const int el = p / NC;
-#pragma unroll 4
-      for (int fr = 0; fr < NF * NFP; ++fr) {
-        const R sv = sS[fr * EPB * NC + p];
-        const R *L = A.Lt + fr * NPAD + i0;
+      for (int w = 0; w < NTW; ++w) acc[m][w] = mxm_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int k0 = 0; k0 < KPAD; k0 += 4) {
+      const int k = k0 + lg;
+      float a[MT];
 #pragma unroll
-        for (int ii = 0; ii < IH; ++ii) acc[m][ii] += L[ii] * sv;
-      }
-      const R niv = -sJ[EPB * ND * ND + el];
+      for (int m = 0; m < MT; ++m) a[m] = A.Aop[(m * 16 + lr) * KPAD + k];
 #pragma unroll
-      for (int ii = 0; ii < IH; ++ii) acc[m][ii] *= niv;
-#pragma unroll 4
-      for (int kj = 0; kj < ND * NP; ++kj) {
-        const R fv = sW[kj * EPB * NC + p];
-        const R *D = A.Dt + kj * NPAD + i0;
+      for (int w = 0; w < NTW; ++w) {
+        const int nt = wave * NTW + w;
+        if (nt < NT) {
+          const int col = nt * 16 + lr;
+          float bv = 0.f;
+          if (col < NCOL) {
+            if (k < KV)
+              bv = float(sW[k * NCOL + col]);
+            else if (k < K)
+              bv = float(sS[(k - KV) * NCOL + col]);
+          }
 #pragma unroll
-        for (int ii = 0; ii < IH; ++ii) acc[m][ii] += D[ii] * fv;
+          for (int m = 0; m < MT; ++m) acc[m][w] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv, acc[m][w], 0, 0, 0);
+        }
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();
 #pragma unroll
-  for (int m = 0; m < ITER; ++m) {
-    const int p = lane + m * GSIZE;
-    if (p < nE * NC) {
-      const int el = p / NC, c = p - el * NC;
+    for (int w = 0; w < NTW; ++w) {
+      const int col = (wave * NTW + w) * 16 + lr;
+      if (wave * NTW + w < NT && col < nE * NC) {
+        const int el = col / NC, c = col - el * NC;
 #pragma unroll
-      for (int ii = 0; ii < IH; ++ii)
-        if (i0 + ii < NP) sW[(el * NP + i0 + ii) * NC + c] = A.cdt * acc[m][ii];
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = m * 16 + 4 * lg + v;
+            if (i < NP) sW[(el * NP + i) * NC + c] = R(acc[m][w][v]);
+          }
+      }
+    }
+  } else {
+    // FP64 on the VALU: the (element, component) columns are the lanes;
+    // each wave group produces IH consecutive output nodes, so the operator
+    // entries are wave-uniform (scalar loads)
+    constexpr int GROUPS = stage_groups(ND, NP, NC);
+    constexpr int GSIZE = S::BLOCK / GROUPS;
+    constexpr int NPAD = stage_node_pad(ND, NP, NC);
+    constexpr int IH = NPAD / GROUPS;
+    constexpr int ITER = (NCOL + GSIZE - 1) / GSIZE;  // columns per thread, at most
+    const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
+    const int i0 = grp * IH;
+    const int lane = tid - grp * GSIZE;
+    double acc[ITER][IH];
+#pragma unroll
+    for (int m = 0; m < ITER; ++m) {
+      const int p = lane + m * GSIZE;
+#pragma unroll
+      for (int ii = 0; ii < IH; ++ii) acc[m][ii] = 0.0;
+      if (p < nE * NC) {
+        const int el = p / NC;
+#pragma unroll 4
+        for (int fr = 0; fr < NF * NFP; ++fr) {
+          const double sv = sS[fr * NCOL + p];
+          const double *L = A.Lt + fr * NPAD + i0;
+#pragma unroll
+          for (int ii = 0; ii < IH; ++ii) acc[m][ii] += L[ii] * sv;
+        }
+        const double niv = -sJ[EPB * ND * ND + el];
+#pragma unroll
+        for (int ii = 0; ii < IH; ++ii) acc[m][ii] *= niv;
+#pragma unroll 4
+        for (int kj = 0; kj < ND * NP; ++kj) {
+          const double fv = sW[kj * NCOL + p];
+          const double *D = A.Dt + kj * NPAD + i0;
+#pragma unroll
+          for (int ii = 0; ii < IH; ++ii) acc[m][ii] += D[ii] * fv;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < ITER; ++m) {
+      const int p = lane + m * GSIZE;
+      if (p < nE * NC) {
+        const int el = p / NC, c = p - el * NC;
+#pragma unroll
+        for (int ii = 0; ii < IH; ++ii)
+          if (i0 + ii < NP) sW[(el * NP + i0 + ii) * NC + c] = R(acc[m][ii]);
+      }
     }
   }
   __syncthreads();
 
-  // ---- 3b. RK combine, limit, store per (element, node) ------------------
+  // ---- 3b. RK combine (FP64), limit, store per (element, node) -----------
   if (node) {
-    R out[NC];
+    double out[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) out[c] = qa[c] + sW[tid * NC + c];
+    for (int c = 0; c < NC; ++c) out[c] = qa[c] + cdt * double(sW[tid * NC + c]);
     if constexpr (Phys::kLimit) phys.limit(out);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      if (c == A.post_comp) out[c] *= A.post_scale;
+      if (c == A.post_comp) out[c] *= post;
     R *g = A.q_out + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) g[c] = out[c];
+    for (int c = 0; c < NC; ++c) g[c] = R(out[c]);
   }
 }
 
-template <class Phys, int ND, int P>
-hipError_t launch_stage(const StageArgs<double> &a, const double *prm, hipStream_t s) {
+template <class Phys, int ND, int P, class R>
+hipError_t launch_stage(const void *args, hipStream_t s) {
   using S = Shape<ND, P>;
-  const Phys ph = Phys::from(prm);
+  const StageArgs<R> &a = *static_cast<const StageArgs<R> *>(args);
   const int nb = a.ntiles;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, double>), dim3(nb), dim3(S::BLOCK), 0, s, a, ph);
+  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R>), dim3(nb), dim3(S::BLOCK), 0, s, a);
   return hipGetLastError();
 }
 
-using StageLaunch = hipError_t (*)(const StageArgs<double> &, const double *, hipStream_t);
-
-// Lookups implemented in the per-family translation units; nullptr when
-// the (physics, variant, P) combination is not compiled.
-StageLaunch find_stage_3d(int phys, int variant, int P);
-StageLaunch find_stage_2d_trait(int phys, int variant, int P);
-StageLaunch find_stage_2d_module(int phys, int variant, int P);
+// args points to StageArgs<double> (FP64 form) or StageArgs<float>
+// (FP32-MFMA form); the tables are in stage_families.hpp.
+using StageLaunch = hipError_t (*)(const void *args, hipStream_t);
 
 }  // namespace mxm

@@ -1,0 +1,6 @@
+// stage2d_module_f32.hip -- Family2DModule stage kernels, FP32-MFMA form, P = 1..5.
+#include "stage_families.hpp"
+
+namespace mxm {
+StageLaunch stage_2d_module_fp32(int phys, int variant, int P) { return pick_order<Family2DModule, float>(phys, variant, P); }
+}  // namespace mxm

@@ -1,0 +1,6 @@
+// stage3d_f32.hip -- Family3D stage kernels, FP32-MFMA form, P = 1..5.
+#include "stage_families.hpp"
+
+namespace mxm {
+StageLaunch stage_3d_fp32(int phys, int variant, int P) { return pick_order<Family3D, float>(phys, variant, P); }
+}  // namespace mxm

@@ -17,6 +17,7 @@ from typing import Callable, Optional
 import numpy as np
 
 from . import physics as ph
+from .boundary import BoundaryConditions2D
 from .time_integrator import apriori_dt, fixed_steps
 
 
@@ -122,3 +123,167 @@ def taylor_green_3d(N=32, P=3, cfl=0.2, nsteps=200, mach=0.3, flux=ph.FLUX_RUSAN
     L = 2.0 * math.pi
     return Problem(f"taylor_green_3d_n{N}_p{P}", (N, N, N), (L, L, L), P, phys,
                    lambda xyz: taylor_green_ic(xyz, mach=mach, gamma=gamma), dt, nsteps)
+
+
+# ---------------------------------------------------------------------------
+# Batched ensemble: linearly polarised Alfven waves with GLM cleaning.
+# ---------------------------------------------------------------------------
+# Directions (a, b) of integer wave vectors 2 pi (a, b), gcd 1, 0 <= angle <
+# pi/2: the waves are periodic on the unit square.
+_ALFVEN_DIRS = [(1, 0), (3, 1), (2, 1), (3, 2), (1, 1), (2, 3), (1, 2), (1, 3)]
+
+
+def alfven_wave(xy: np.ndarray, t: float, a: int, b: int, amp: float, gamma=5.0 / 3.0, rho0=1.0, p0=0.1,
+                B0=1.0) -> np.ndarray:
+    """Linear Alfven wave travelling along k = 2 pi (a, b) with the
+    background field B0 along k (examples/mhd_alfven_glm_2d_gpu.mojo:98-115
+    rotated): u = A sin(phi) t_hat, B = B0 k_hat - A sqrt(rho0) sin(phi) t_hat,
+    phi = k . x - |k| c_A t, c_A = B0 / sqrt(rho0); psi = 0.  NC = 7:
+    (rho, rho u, rho v, Bx, By, E, psi)."""
+    kn = math.hypot(a, b)
+    kx, ky = a / kn, b / kn
+    tx, ty = -ky, kx
+    cA = B0 / math.sqrt(rho0)
+    phi = 2.0 * math.pi * (a * xy[..., 0] + b * xy[..., 1]) - 2.0 * math.pi * kn * cA * t
+    s = amp * np.sin(phi)
+    u, v = s * tx, s * ty
+    Bx = B0 * kx - math.sqrt(rho0) * s * tx
+    By = B0 * ky - math.sqrt(rho0) * s * ty
+    rho = np.full_like(s, rho0)
+    E = p0 / (gamma - 1.0) + 0.5 * rho * (u * u + v * v) + 0.5 * (Bx * Bx + By * By)
+    return np.stack([rho, rho * u, rho * v, Bx, By, E, np.zeros_like(s)], axis=-1)
+
+
+@dataclass
+class EnsembleMember:
+    """One instance of the batched ensemble."""
+    index: int
+    n: tuple
+    L: tuple
+    P: int
+    physics: object
+    dt: float
+    nsteps: int
+    ic: Callable[[np.ndarray], np.ndarray]
+    exact: Callable[[np.ndarray, float], np.ndarray]
+    params: dict = field(default_factory=dict)
+
+
+def alfven_ensemble(count=512, seed=0, P=3, t_final=1.0, cfl=0.15, cr=0.18, n_min=16, n_max=96):
+    """The seeded batched ensemble: `count` 2D ideal-MHD + GLM (NC = 7)
+    Alfven-wave problems on periodic unit squares, P = 3.  Per instance:
+    mesh side n = round(n_min (n_max/n_min)^u), u ~ U[0,1) (log-uniform
+    integer in [n_min, n_max], 2 n^2 triangles); wave angle ~ U[0, pi/2),
+    taken to the nearest direction of _ALFVEN_DIRS so that the wave is
+    periodic; amplitude ~ U[0.01, 0.2].  Background rho = 1, p = 0.1,
+    B0 = 1, gamma = 5/3; c_h = the largest signal speed of the initial
+    state, alpha_d = c_h / cr; dt = CFL h / (c_h (2P+1)), steps to T."""
+    rng = np.random.default_rng(seed)
+    gamma, rho0, p0, B0 = 5.0 / 3.0, 1.0, 0.1, 1.0
+    dirs = np.array([math.atan2(b, a) for a, b in _ALFVEN_DIRS])
+    out = []
+    for i in range(count):
+        n = int(round(n_min * (n_max / n_min) ** rng.uniform(0.0, 1.0)))
+        theta = rng.uniform(0.0, 0.5 * math.pi)
+        amp = rng.uniform(0.01, 0.2)
+        a, b = _ALFVEN_DIRS[int(np.argmin(np.abs(dirs - theta)))]
+        # fast speed bound: |u| + sqrt(cs^2 + |B|^2 / rho)
+        c_h = amp + math.sqrt(gamma * p0 / rho0 + (B0 * B0 + amp * amp) / rho0)
+        phys = ph.MHDGLM2D(gamma=gamma, c_h=c_h, alpha_d=c_h / cr)
+        nsteps, dt = fixed_steps(t_final, apriori_dt(cfl, 1.0 / n, c_h, P))
+        kw = dict(a=a, b=b, amp=amp, gamma=gamma, rho0=rho0, p0=p0, B0=B0)
+        out.append(EnsembleMember(i, (n, n), (1.0, 1.0), P, phys, dt, nsteps,
+                                  lambda xy, kw=kw: alfven_wave(xy, 0.0, **kw),
+                                  lambda xy, t, kw=kw: alfven_wave(xy, t, **kw),
+                                  dict(theta=theta, direction=(a, b), amplitude=amp, c_h=c_h)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Serving stream: a seeded Poisson stream of three request types.
+# ---------------------------------------------------------------------------
+@dataclass
+class Request:
+    """One solve request of the serving stream."""
+    index: int
+    kind: str
+    arrival: float  # seconds after the stream starts
+    n: tuple
+    L: tuple
+    P: int
+    physics: object
+    bcs: object
+    limiter: bool
+    eps: float
+    dt: float
+    nsteps: int
+    ic: Callable[[np.ndarray], np.ndarray]
+    params: dict = field(default_factory=dict)
+
+
+def dam_break_ic(xy, ratio, nc=3):
+    """Shallow-water dam break at x = 0.5: h = ratio left, 1 right, at rest
+    (examples/shallow_water_dam_break_2d_gpu.mojo:70-80)."""
+    h = np.where(xy[..., 0] < 0.5, ratio, 1.0)
+    return np.stack([h, np.zeros_like(h), np.zeros_like(h)], axis=-1)
+
+
+def sod_ic(xy, n, gamma=1.4):
+    """Sod tube along x on the unit square, (rho, p) = (1, 1) | (0.125, 0.1),
+    tanh-smoothed over 8 cells (examples/euler_sod_2d_gpu.mojo:86-104);
+    trait Euler state (rho, rhou, rhov, rhow, E)."""
+    s = 0.5 * (np.tanh((xy[..., 0] - 0.5) / (8.0 / n)) + 1.0)
+    rho = 1.0 + s * (0.125 - 1.0)
+    p = 1.0 + s * (0.1 - 1.0)
+    z = np.zeros_like(rho)
+    return np.stack([rho, z, z, z, p / (gamma - 1.0)], axis=-1)
+
+
+def cavity_ic(xy, m, k):
+    """TM(m, k) mode of the PEC unit square at t = 0: Ez = sin(m pi x)
+    sin(k pi y), B = 0 (examples/maxwell_cavity_2d_gpu.mojo:8-10)."""
+    ez = np.sin(m * math.pi * xy[..., 0]) * np.sin(k * math.pi * xy[..., 1])
+    z = np.zeros_like(ez)
+    return np.stack([z, z, ez, z, z, z], axis=-1)
+
+
+def serving_stream(count=64, rate=20.0, seed=0, n_min=16, n_max=256, orders=(2, 3, 4), t_range=(0.1, 1.0)):
+    """The seeded serving stream: Poisson arrivals at `rate` requests/s;
+    each request uniformly one of a shallow-water dam break (height ratio
+    U[2,10], slip walls, HLL + limiter), an Euler Sod tube (HLLE + limiter,
+    outflow x, walls y) or a Maxwell PEC cavity mode (m, k) in {1..4}^2;
+    mesh side n ~ U{n_min..n_max} (2 n^2 triangles on the unit square),
+    P ~ U(orders), end time ~ U(t_range) = U[0.1, 1]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate)
+        kind = ("dam_break", "sod", "cavity")[int(rng.integers(0, 3))]
+        n = int(rng.integers(n_min, n_max + 1))
+        P = int(orders[int(rng.integers(0, len(orders)))])
+        T = float(rng.uniform(*t_range))
+        h = 1.0 / n
+        if kind == "dam_break":
+            ratio = float(rng.uniform(2.0, 10.0))
+            phys = ph.ShallowWater2D(g=9.81, hll=True)
+            bcs = BoundaryConditions2D(1, 1, 1, 1)
+            nsteps, dt = fixed_steps(T, apriori_dt(0.2, h, math.sqrt(9.81 * ratio), P))
+            ic = (lambda xy, r=ratio: dam_break_ic(xy, r))
+            prm, lim = dict(ratio=ratio), True
+        elif kind == "sod":
+            phys = ph.Euler(gamma=1.4, flux_type=ph.FLUX_HLLE)
+            bcs = BoundaryConditions2D(2, 2, 1, 1)
+            nsteps, dt = fixed_steps(T, apriori_dt(0.15, h, 2.0 * math.sqrt(1.4), P))
+            ic = (lambda xy, n=n: sod_ic(xy, n))
+            prm, lim = {}, True
+        else:
+            m, k = int(rng.integers(1, 5)), int(rng.integers(1, 5))
+            phys = ph.Maxwell2D(c=1.0)
+            bcs = BoundaryConditions2D(1, 1, 1, 1)
+            nsteps, dt = fixed_steps(T, apriori_dt(0.2, h, 1.0, P))
+            ic = (lambda xy, m=m, k=k: cavity_ic(xy, m, k))
+            prm, lim = dict(mode=(m, k)), False
+        prm.update(t_final=T)
+        out.append(Request(i, kind, t, (n, n), (1.0, 1.0), P, phys, bcs, lim, 0.1, dt, nsteps, ic, prm))
+    return out

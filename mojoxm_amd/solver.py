@@ -12,7 +12,8 @@ from . import _lib as L
 from .boundary import BoundaryConditions, BoundaryConditions2D
 from .memory_report import MemoryReport, ThroughputReport
 
-PREC_FP64 = 0
+PREC_FP64 = 0       # plain FP64 (include/mxm_dg.h MXM_PREC_FP64)
+PREC_FP32_MFMA = 1  # FP32 state and fluxes, volume + LIFT on FP32 MFMA (MXM_PREC_FP32_MFMA)
 
 
 class Solver:
@@ -21,7 +22,9 @@ class Solver:
 
     Parameters mirror the reference's Driver3D / 2D driver composition:
     mesh size and extent, boundary conditions, order P and a physics
-    descriptor from ``mojoxm_amd.physics``.
+    descriptor from ``mojoxm_amd.physics``.  ``precision`` selects the
+    plain FP64 kernels (PREC_FP64) or the FP32-MFMA form (PREC_FP32_MFMA);
+    the host side always sees FP64 arrays.
     """
 
     def __init__(self, n, L_box, physics, P: int = 2, bcs=None, precision: int = PREC_FP64, device: int = 0):
@@ -30,6 +33,7 @@ class Solver:
         if self.ndim not in physics.dims:
             raise ValueError(f"{type(physics).__name__} does not run on a {self.ndim}D mesh")
         self.physics, self.P, self.n, self.L = physics, P, tuple(n), tuple(L_box)
+        self.precision = precision
         self._prm = np.ascontiguousarray(physics.params(), dtype=np.float64)
         h = C.c_void_p()
         if self.ndim == 3:

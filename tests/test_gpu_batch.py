@@ -1,0 +1,117 @@
+"""The batch engine (mxm_batch_*): every instance of a batch must give the
+same state, bit for bit, as the same problem advanced alone by a Solver
+with the same arithmetic form -- whatever the other instances, their step
+counts, and instances joining or leaving mid-run.  The Solver path is tied
+to the FP64 oracle by test_gpu_parity.py, so this pins the batch to the
+oracle as well; one instance is also checked against the oracle directly.
+Each batched step replays a HIP graph, so the multi-step comparisons
+check every replay against the eager single-solver launches."""
+import numpy as np
+import pytest
+
+import mojoxm_amd as M
+from mojoxm_amd import physics as ph
+from mojoxm_amd.batch import Batch, footprint
+
+from test_gpu_parity import random_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def alone(n, L, phys, P, q0, dt, nsteps, precision, bcs=None, limiter=False, eps=0.1):
+    with M.Solver(n, L, phys, P=P, bcs=bcs, precision=precision) as s:
+        s.set_state(q0)
+        if limiter:
+            s.enable_cell_limiter(True, eps)
+        s.step_ssprk3(dt, nsteps)
+        return s.get_state()
+
+
+def make_cases(rng):
+    """(n, L, physics, P, dt, nsteps, bcs, limiter) of a mixed batch: three
+    kernel groups, different sizes, parameters and step counts."""
+    cases = []
+    for k, nn in enumerate([(6, 4), (9, 7), (4, 4)]):
+        cases.append((nn, (1.0, 0.9), ph.Euler(flux_type=ph.FLUX_HLLEC, gamma=1.4 - 0.05 * k), 3, 2e-3 * (k + 1),
+                      3 + 2 * k, None, False))
+    for k, nn in enumerate([(8, 3), (5, 5)]):
+        cases.append((nn, (1.0, 1.0), ph.MHDGLM2D(c_h=1.2 + 0.2 * k, alpha_d=(1.2 + 0.2 * k) / 0.18), 2, 1.5e-3,
+                       4 + k, None, False))
+    cases.append(((3, 2, 2), (1.0, 1.0, 1.0), ph.Advection(1.0, -0.5, 0.3), 2, 3e-3, 5, None, False))
+    cases.append(((10, 3), (1.0, 0.3), ph.Euler(flux_type=ph.FLUX_HLLE), 2, 1e-3, 4,
+                  M.BoundaryConditions2D(2, 2, 1, 1), True))
+    return cases
+
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_FP32_MFMA])
+def test_batch_matches_alone(precision):
+    rng = np.random.default_rng(3)
+    cases = make_cases(rng)
+    states = []
+    with Batch(precision) as b:
+        ids = []
+        for (n, L, phys, P, dt, ns, bcs, lim) in cases:
+            K = int(np.prod(n)) * (6 if len(n) == 3 else 2)
+            NP = (P + 1) * (P + 2) * (P + 3) // 6 if len(n) == 3 else (P + 1) * (P + 2) // 2
+            q0 = random_state(phys, K, NP, rng, len(n))
+            if precision == M.PREC_FP32_MFMA:
+                q0 = q0.astype(np.float32).astype(np.float64)
+            states.append(q0)
+            ids.append(b.add(n, L, phys, P, q0, dt, ns, bcs=bcs, limiter=lim))
+        done = []
+        for _ in range(10):
+            b.step(1)
+            done += b.poll()
+        assert sorted(done) == sorted(ids)
+        for k, (n, L, phys, P, dt, ns, bcs, lim) in enumerate(cases):
+            assert b.status(ids[k]) == ns
+            got = b.get_state(ids[k])
+            want = alone(n, L, phys, P, states[k], dt, ns, precision, bcs=bcs, limiter=lim)
+            assert np.array_equal(got, want), (k, rel_l2(got, want))
+
+
+def test_batch_joins_and_leaves():
+    """An instance added mid-run, and one removed mid-run, leave the others
+    (and themselves) unchanged."""
+    rng = np.random.default_rng(5)
+    phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
+    n, L, P, dt = (6, 6), (1.0, 1.0), 3, 2e-3
+    q = [random_state(phys, 72, 10, rng, 2) for _ in range(3)]
+    with Batch(M.PREC_FP64) as b:
+        a = b.add(n, L, phys, P, q[0], dt, 6)
+        c = b.add(n, L, phys, P, q[1], dt, 6)
+        b.step(2)
+        d = b.add(n, L, phys, P, q[2], dt, 3)  # joins after 2 steps of a, c
+        b.remove(c)
+        b.step(4)
+        assert sorted(b.poll()) == sorted([a, d])
+        assert np.array_equal(b.get_state(a), alone(n, L, phys, P, q[0], dt, 6, M.PREC_FP64))
+        assert np.array_equal(b.get_state(d), alone(n, L, phys, P, q[2], dt, 3, M.PREC_FP64))
+
+
+def test_batch_against_oracle(oracle):
+    rng = np.random.default_rng(9)
+    phys = ph.MHDGLM2D(c_h=1.3, alpha_d=1.3 / 0.18)
+    n, L, P, dt = (5, 4), (1.0, 1.0), 3, 1e-3
+    q0 = random_state(phys, 40, 10, rng, 2)
+    with Batch(M.PREC_FP64) as b:
+        i = b.add(n, L, phys, P, q0, dt, 3)
+        b.add((7, 7), L, phys, P, random_state(phys, 98, 10, rng, 2), dt, 5)
+        b.step(5)
+        got = b.get_state(i)
+    want = oracle.Mesh(2, P, n, L).step(phys.phys_id, phys.params(), q0, dt, 3)
+    assert rel_l2(got, want) < 1e-12
+
+
+def test_batch_memory_matches_footprints():
+    phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
+    with Batch(M.PREC_FP32_MFMA) as b:
+        total = 0
+        for nn in [(8, 8), (12, 6)]:
+            K = nn[0] * nn[1] * 2
+            b.add(nn, (1.0, 1.0), phys, 3, np.ones((K, 10, 5)), 1e-3, 1)
+            total += footprint(nn, (1.0, 1.0), phys, 3, M.PREC_FP32_MFMA)
+        b.step(1)
+        used = b.memory()
+    # the group adds its operators and a shared-stride padding of the job tables
+    assert total <= used <= total + 64 * 1024
