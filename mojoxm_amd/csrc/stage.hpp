@@ -135,11 +135,12 @@ struct FaceItem {
 };
 
 // Every record of face item t of a tile is addressed from the tile index,
-// so all of them are in flight together; the partner trace (outside the
-// tile) is the only load that waits for another.
-template <int ND, int NP, int NFP, int NC, class R>
-__device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int tile, int t, int e0, int nE,
-                                               FaceItem<R, NC> &it) {
+// so all of them are in flight together (load_item_records); the partner
+// trace of a face to another tile is the only load that waits for one of
+// them (load_item_trace).
+template <int NFP, class R, int NC>
+__device__ __forceinline__ void load_item_records(const StageArgs<R> &A, int tile, int t, int e0,
+                                                  FaceItem<R, NC> &it) {
   const int jb = t / NFP;
   it.r = t - jb * NFP;
   const size_t j = (size_t)tile * A.maxj + jb;
@@ -155,9 +156,13 @@ __device__ __forceinline__ void load_face_item(const StageArgs<R> &A, int tile, 
   it.pn = pr & 255;
   it.rs = pr >> 8;
   it.pl = jr.y - e0;
-  const bool need = jr.x >= 0 && !(it.side == 0 && it.bc != 0);
+}
+
+template <int NP, class R, int NC>
+__device__ __forceinline__ void load_item_trace(const StageArgs<R> &A, int e0, int nE, FaceItem<R, NC> &it) {
+  const bool need = it.packed >= 0 && !(it.side == 0 && it.bc != 0);
   if (need && !(it.pl >= 0 && it.pl < nE)) {
-    const R *g = A.q_in + ((size_t)jr.y * NP + it.pn) * NC;
+    const R *g = A.q_in + ((size_t)(e0 + it.pl) * NP + it.pn) * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) it.qp[c] = g[c];
   }
@@ -195,24 +200,59 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A) {
   // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
   const bool node = tid < nE * NP;
 
-  // ---- 1. first face item of this thread, then the tile ------------------
-  FaceItem<R, NC> it;
-  if (tid < nitems) load_face_item<ND, NP, NFP, NC>(A, tile, tid, e0, nE, it);
+  // ---- 1. loads --------------------------------------------------------
+  // First everything addressed from the tile index -- the q_in tile, the
+  // geometry, the first face item's records and q_a of the own node --
+  // then the partner trace that waits for the records; the tile and the
+  // geometry go to LDS last.
+  constexpr int QPT = (TILE + S::BLOCK - 1) / S::BLOCK;  // tile values per thread
+  constexpr int NJ = EPB * ND * ND, NG = NJ + EPB;      // invJ, then inv_vol
+  constexpr int GPT = (NG + S::BLOCK - 1) / S::BLOCK;
+  const int nq = nE * NP * NC;
+  R tq[QPT];
   {
     const R *src = A.q_in + (size_t)e0 * NP * NC;
-    const int n = nE * NP * NC;
-    for (int t = tid; t < n; t += S::BLOCK) sW[t] = src[t];
-    const double *gJ = A.invJ + (size_t)e0 * ND * ND;
-    for (int t = tid; t < nE * ND * ND; t += S::BLOCK) sJ[t] = gJ[t];
-    for (int t = tid; t < nE; t += S::BLOCK) sJ[EPB * ND * ND + t] = A.inv_vol[e0 + t];
-    for (int t = tid; t < NF * NFP; t += S::BLOCK) sMap[t] = A.fmap[t];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k)
+      if (k * S::BLOCK + tid < nq) tq[k] = src[k * S::BLOCK + tid];
   }
+  double tg[GPT];
+#pragma unroll
+  for (int k = 0; k < GPT; ++k) {
+    const int t = k * S::BLOCK + tid;
+    if (t < nE * ND * ND)
+      tg[k] = A.invJ[(size_t)e0 * ND * ND + t];
+    else if (t >= NJ && t < NJ + nE)
+      tg[k] = A.inv_vol[e0 + t - NJ];
+  }
+  const int fm = tid < NF * NFP ? A.fmap[tid] : 0;
+  FaceItem<R, NC> it;
+  if (tid < nitems) load_item_records<NFP>(A, tile, tid, e0, it);
+  R qa_in[NC];  // q_a of the own node, in the state precision until 2b
+  if (node) {
+    const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qa_in[c] = g[c];
+  }
+  if (tid < nitems) load_item_trace<NP>(A, e0, nE, it);
+#pragma unroll
+  for (int k = 0; k < QPT; ++k)
+    if (k * S::BLOCK + tid < nq) sW[k * S::BLOCK + tid] = tq[k];
+#pragma unroll
+  for (int k = 0; k < GPT; ++k) {
+    const int t = k * S::BLOCK + tid;
+    if (t < nE * ND * ND || (t >= NJ && t < NJ + nE)) sJ[t] = tg[k];
+  }
+  if (tid < NF * NFP) sMap[tid] = fm;
   __syncthreads();
   const R *sQ = sW;  // the q_in tile, [el][j][c], until the flux planes are written
 
   // ---- 2a. face fluxes ----------------------------------------------------
   for (int t = tid; t < nitems; t += S::BLOCK) {
-    if (t != tid) load_face_item<ND, NP, NFP, NC>(A, tile, t, e0, nE, it);
+    if (t != tid) {
+      load_item_records<NFP>(A, tile, t, e0, it);
+      load_item_trace<NP>(A, e0, nE, it);
+    }
     if (it.packed < 0) continue;
     const int el = it.packed & 127, lf = (it.packed >> 7) & 3, pin = (it.packed >> 9) & 127;
     R qm[NC], qo[NC], ql[NC], qr[NC];
@@ -251,20 +291,12 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A) {
     }
   }
 
-  // q_a of the own node: loaded only now, so its registers are not held
-  // through the face phase (the register peak)
-  double qa[NC];
-  if (node) {
-    const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) qa[c] = double(g[c]);
-  }
-
   // ---- 2b. nodal flux of the own node, and the RK terms of q_in ----------
   // qa becomes a q_a + b q_in + c dt S(q_in) (FP64); the flux, mapped to
   // reference directions by invJ in FP64, waits in registers until every
   // read of the q_in tile is done.
   R Fk[ND * NC];
+  double qa[NC];
   if (node) {
     const int el = tid / NP;
     const R *q = &sQ[tid * NC];
@@ -287,7 +319,7 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A) {
       if (phys.source_active()) phys.source(q, src);
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) qa[c] = A.a * qa[c] + A.b * double(q[c]) + cdt * double(src[c]);
+    for (int c = 0; c < NC; ++c) qa[c] = A.a * double(qa_in[c]) + A.b * double(q[c]) + cdt * double(src[c]);
   }
   __syncthreads();
   if (node) {
