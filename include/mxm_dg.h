@@ -132,7 +132,8 @@ typedef struct {
 int mxm_hip_init(int device);
 const char *mxm_last_error(void);
 /* Constants by getter, as src/mpi_shim.c:148-160 does for MPI handles:
- * which = 0 -> MXM_PRM_MAX, 1 -> number of device kernels compiled in. */
+ * which = 0 -> MXM_PRM_MAX, 1 -> number of stage kernels compiled in,
+ * 2 / 3 -> launch configurations of the FP64 / FP32-MFMA stage kernels. */
 int mxm_get_constant(int which, int *value);
 
 /* ---- host-only builders (no GPU needed) ---------------------------------- */
@@ -186,6 +187,18 @@ int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
  * own stream around them; advances the state.  No reference counterpart:
  * it is the measurement hook that the roofline figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
+/* Launch configurations (no reference counterpart; the reference has one
+ * launch shape per kernel).  Each configuration of the stage kernel is an
+ * occupancy target (waves per SIMD) that bounds its register allocation;
+ * all compute the same bits.  mxm_dg_autotune times each on this solver's
+ * mesh -- reps launches of SSPRK3 stage 1, which writes a work buffer and
+ * leaves the state unchanged -- and keeps the fastest: *cfg is its index,
+ * us[0..cap) the microseconds per launch of each configuration
+ * (mxm_get_constant(2 / 3) = number of configurations of the FP64 /
+ * FP32-MFMA form).  mxm_dg_launch_config sets configuration cfg (cfg < 0:
+ * no change) and reports the current one. */
+int mxm_dg_autotune(void *solver, double dt, int reps, int *cfg, double *us, int cap);
+int mxm_dg_launch_config(void *solver, int cfg, int *current);
 int mxm_dg_destroy(void *solver);
 
 /* ---- batch engine ----------------------------------------------------------
@@ -200,6 +213,11 @@ int mxm_dg_destroy(void *solver);
  * not depend on which other instances are in the batch.  The precision of
  * the batch applies to every instance (MXM_PREC_*). */
 int mxm_batch_create(int precision, void **batch);
+/* Autotuning of the stage kernels' launch configuration (on by default):
+ * when a kernel group is packed, each configuration (see mxm_dg_autotune)
+ * is timed on the group's arena and the fastest kept, once per group and
+ * power-of-two range of its tile count. */
+int mxm_batch_autotune(void *batch, int on);
 /* Queue an instance: mesh, order, physics (id + prm[MXM_PRM_MAX]), time
  * step dt, step count, limiter on/off with its eps, initial state q0
  * [K*NP*NC] (reference order, FP64).  It joins at the next

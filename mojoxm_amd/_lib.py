@@ -69,8 +69,12 @@ EXPORTS = {
     "mxm_dg_state_bytes_per_step": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),
     "mxm_dg_memory_report": ([C.c_void_p, C.POINTER(MemReport)], C.c_int),
     "mxm_dg_time_stage_kernel": ([C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_double)], C.c_int),
+    "mxm_dg_autotune": ([C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
+                        C.c_int),
+    "mxm_dg_launch_config": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_destroy": ([C.c_void_p], C.c_int),
     "mxm_batch_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "mxm_batch_autotune": ([C.c_void_p, C.c_int], C.c_int),
     "mxm_batch_add_2d": ([C.c_void_p, C.POINTER(Mesh2DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,
                           C.c_int, C.c_double, C.c_void_p, C.c_longlong, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_add_3d": ([C.c_void_p, C.POINTER(Mesh3DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,

@@ -66,6 +66,11 @@ class Batch:
     def __exit__(self, *exc):
         self.close()
 
+    def autotune(self, on: bool = True) -> None:
+        """Autotune the stage kernels' launch configuration per kernel group
+        and tile-count range when groups are packed (on by default)."""
+        L.check(L.lib().mxm_batch_autotune(self._h, int(bool(on))), "mxm_batch_autotune")
+
     def add(self, n, L_box, physics, P, q0, dt, nsteps, bcs=None, limiter=False, eps=0.1) -> int:
         """Queue one instance; it joins at the next step().  Returns its id."""
         ndim = len(n)

@@ -58,7 +58,8 @@ struct BInst {
 using GroupKey = std::tuple<int, int, int, int, int, long long>;  // dim, P, phys, variant, limiter, eps bits
 
 struct BGroup {
-  int dim = 0, P = 0, phys = 0, NC = 0, NP = 0, NFP = 0, NF = 0, epb = 0, prec = 0;
+  int dim = 0, P = 0, phys = 0, variant = 0, NC = 0, NP = 0, NFP = 0, NF = 0, epb = 0, prec = 0;
+  int cfg = 0;  // launch configuration (StageCfg), autotuned per packed size
   bool limiter = false;
   double eps = 0.1;
   StageLaunch launch = nullptr;
@@ -75,6 +76,7 @@ struct BGroup {
   DevBuf<int> fmap, tile_count, tile_inst, iactive, isteps, insteps;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  GroupKey key{};
 
   ~BGroup() { drop_graph(); }
   void drop_graph() {
@@ -115,6 +117,9 @@ struct Batch {
   std::map<int, std::unique_ptr<BInst>> insts;
   std::map<std::pair<int, int>, RefElem> refs;
   int next_id = 0;
+  // autotuned launch configuration per (group, tile-count octave)
+  bool tune = true;
+  std::map<std::pair<GroupKey, int>, int> tuned;
 
   ~Batch() {
     groups.clear();
@@ -136,11 +141,14 @@ struct Batch {
     if (it != groups.end()) return *it->second;
     auto g = std::make_unique<BGroup>();
     const RefElem &re = ref(in.dim, in.P);
-    g->dim = in.dim, g->P = in.P, g->phys = in.phys, g->NC = in.NC, g->NP = re.NP, g->NFP = re.NFP, g->NF = re.NF;
+    g->dim = in.dim, g->P = in.P, g->phys = in.phys, g->variant = in.variant, g->NC = in.NC, g->NP = re.NP,
+    g->NFP = re.NFP, g->NF = re.NF;
     g->epb = stage_epb(in.dim, re.NP);
     g->prec = prec;
     g->limiter = in.limiter, g->eps = in.eps;
-    g->launch = find_stage(in.dim, in.phys, in.variant, in.P, prec);
+    g->key = key;
+    g->cfg = stage_cfg_default(prec);
+    g->launch = find_stage(in.dim, in.phys, in.variant, in.P, prec, g->cfg);
     if (!g->launch) throw ArgError("no stage kernel for this (physics, flux, P)");
     OpTables o = build_operators(re, in.dim, in.NC, prec);
     if (prec == MXM_PREC_FP64) {
@@ -325,26 +333,59 @@ struct Batch {
     if (g.limiter) g.mean.alloc(slots * g.NC);
     g.packed = g.members;
     g.dirty = false;
-    if (g.ntiles > 0) capture(g);
+    if (g.ntiles > 0) {
+      if (tune) choose_config(g);
+      capture(g);
+    }
+  }
+
+  // the launch configuration for the group's packed size: from the cache
+  // of this (group, tile-count octave), else timed on stage 1 (q -> q1,
+  // which the next step rewrites before reading)
+  void choose_config(BGroup &g) {
+    int oct = 0;
+    while ((1 << oct) < g.ntiles) ++oct;
+    const auto key = std::make_pair(g.key, oct);
+    auto it = tuned.find(key);
+    if (it == tuned.end()) {
+      MXM_HIP(hipStreamSynchronize(stream));
+      int c;
+      if (g.prec == MXM_PREC_FP64) {
+        const StageArgs<double> a = stage_args(g, g.qd, 0);
+        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, stream, 10, nullptr);
+      } else {
+        const StageArgs<float> a = stage_args(g, g.qf, 0);
+        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, stream, 10, nullptr);
+      }
+      it = tuned.emplace(key, c).first;
+    }
+    g.cfg = it->second;
+    g.launch = find_stage(g.dim, g.phys, g.variant, g.P, g.prec, g.cfg);
+  }
+
+  template <class R>
+  StageArgs<R> stage_args(const BGroup &g, DevBuf<R> (&q)[3], int s) const {
+    StageArgs<R> a{};
+    const StagePlan pl = stage_plan(s);
+    a.q_in = q[pl.in].p, a.q_a = q[pl.qa].p, a.q_out = q[pl.out].p;
+    a.a = pl.a, a.b = pl.b, a.c = pl.c;
+    a.tile_inst = g.tile_inst.p, a.inst_prm = g.iprm.p, a.inst_dt = g.idt.p, a.inst_post = g.ipost.p;
+    a.inst_active = g.iactive.p;
+    a.dt = 0.0, a.post_scale = 1.0;
+    // GLM decay per instance (factor 1 where alpha_d = 0)
+    a.post_comp = s == 2 && g.phys == MXM_PHYS_MHDGLM2D ? 6 : -1;
+    a.tile_count = g.tile_count.p;
+    a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p;
+    a.Dt = g.Dt.p, a.Lt = g.Lt.p, a.Aop = g.Aop.p, a.fmap = g.fmap.p;
+    a.jobs = g.jobs.p, a.jgeo = g.jgeo.p, a.ipr = g.ipr.p, a.maxj = g.maxj, a.ntiles = g.ntiles;
+    return a;
   }
 
   template <class R>
   void launch_step(BGroup &g, DevBuf<R> (&q)[3]) {
     const int ni = (int)g.members.size();
     for (int s = 0; s < 3; ++s) {
-      StageArgs<R> a{};
-      const StagePlan pl = stage_plan(s);
-      a.q_in = q[pl.in].p, a.q_a = q[pl.qa].p, a.q_out = q[pl.out].p;
-      a.a = pl.a, a.b = pl.b, a.c = pl.c;
-      a.tile_inst = g.tile_inst.p, a.inst_prm = g.iprm.p, a.inst_dt = g.idt.p, a.inst_post = g.ipost.p;
-      a.inst_active = g.iactive.p;
-      a.dt = 0.0, a.post_scale = 1.0;
-      // GLM decay per instance (factor 1 where alpha_d = 0)
-      a.post_comp = s == 2 && g.phys == MXM_PHYS_MHDGLM2D ? 6 : -1;
-      a.tile_count = g.tile_count.p;
-      a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p;
-      a.Dt = g.Dt.p, a.Lt = g.Lt.p, a.Aop = g.Aop.p, a.fmap = g.fmap.p;
-      a.jobs = g.jobs.p, a.jgeo = g.jgeo.p, a.ipr = g.ipr.p, a.maxj = g.maxj, a.ntiles = g.ntiles;
+      const StageArgs<R> a = stage_args(g, q, s);
       MXM_HIP(g.launch(&a, stream));
       if (g.limiter)
         MXM_HIP(launch_limiter<R>(a.q_out, g.mean.p, g.w.p, g.conn.p, g.ntiles * g.epb, g.NP, g.NC, g.NF, g.dim,
@@ -480,6 +521,13 @@ int mxm_batch_create(int precision, void **out) {
     b->prec = precision;
     MXM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     *out = b.release();
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_autotune(void *h, int on) {
+  return guarded([&] {
+    as_batch(h)->tune = on != 0;
     return MXM_OK;
   });
 }

@@ -99,11 +99,49 @@ inline int phys_variant(int phys, const double *prm) {
   return 0;
 }
 
-inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec) {
+// launch configurations of each arithmetic form (StageCfg in stage.hpp)
+inline int stage_cfg_count(int prec) { return prec == MXM_PREC_FP64 ? StageCfg<double>::count : StageCfg<float>::count; }
+inline int stage_cfg_default(int prec) { return prec == MXM_PREC_FP64 ? StageCfg<double>::dflt : StageCfg<float>::dflt; }
+
+inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec, int cfg) {
   const bool f64 = prec == MXM_PREC_FP64;
-  if (dim == 3) return phys <= MXM_PHYS_EULER ? (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P) : nullptr;
-  if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P);
-  return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P);
+  if (cfg < 0 || cfg >= stage_cfg_count(prec)) return nullptr;
+  if (dim == 3) return phys <= MXM_PHYS_EULER ? (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg) : nullptr;
+  if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P, cfg);
+  return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P, cfg);
+}
+
+// Times `launch` over `reps` back-to-back launches (after one untimed
+// launch) with two events on `stream`; returns microseconds per launch.
+inline double time_launches(StageLaunch launch, const void *args, hipStream_t stream, int reps) {
+  hipEvent_t ev[2];
+  for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
+  MXM_HIP(launch(args, stream));
+  MXM_HIP(hipEventRecord(ev[0], stream));
+  for (int k = 0; k < reps; ++k) MXM_HIP(launch(args, stream));
+  MXM_HIP(hipEventRecord(ev[1], stream));
+  MXM_HIP(hipEventSynchronize(ev[1]));
+  float ms = 0.f;
+  MXM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return (double)ms * 1e3 / reps;
+}
+
+// The autotuner: times every launch configuration of one stage launch
+// (its arguments must leave the launch's inputs unchanged, e.g. stage 1
+// of SSPRK3, which writes q1 from q) and returns the fastest; us[k] gets
+// configuration k's time when us is not null.
+inline int autotune_stage(int dim, int phys, int variant, int P, int prec, const void *args, hipStream_t stream,
+                          int reps, double *us) {
+  int best = stage_cfg_default(prec);
+  double tbest = 0.0;
+  for (int c = 0; c < stage_cfg_count(prec); ++c) {
+    StageLaunch l = find_stage(dim, phys, variant, P, prec, c);
+    const double t = time_launches(l, args, stream, reps);
+    if (us) us[c] = t;
+    if (c == 0 || t < tbest) best = c, tbest = t;
+  }
+  return best;
 }
 
 inline void check_bc(const int *bc, int n) {

@@ -25,6 +25,7 @@ struct Solver {
   int K = 0, Nf = 0;
   double prm[MXM_PRM_MAX] = {0};
   int prec = MXM_PREC_FP64;
+  int variant = 0, cfg = 0;  // flux variant; launch configuration (StageCfg)
   StageLaunch launch = nullptr;
   hipStream_t stream = nullptr;
   bool limiter = false;
@@ -100,7 +101,7 @@ struct Solver {
   }
 
   template <class R>
-  void stage_with(int s, double dt, R *b0, R *b1, R *b2) {
+  StageArgs<R> stage_args(int s, double dt, R *b0, R *b1, R *b2) const {
     StageArgs<R> a{};
     R *trio[3] = {b0, b1, b2};
     const StagePlan pl = stage_plan(s);
@@ -114,8 +115,29 @@ struct Solver {
     a.jobs = jobs.p, a.jgeo = jgeo.p, a.ipr = ipr.p, a.maxj = maxj, a.ntiles = ntiles;
     a.post_comp = s == 2 ? post_component(phys, prm) : -1;
     a.post_scale = post_factor(phys, prm, dt);
+    return a;
+  }
+
+  template <class R>
+  void stage_with(int s, double dt, R *b0, R *b1, R *b2) {
+    const StageArgs<R> a = stage_args(s, dt, b0, b1, b2);
     MXM_HIP(launch(&a, stream));
     if (limiter) MXM_HIP(launch_limiter<R>(a.q_out, mean.p, w.p, conn.p, Kd, NP, NC, NF, dim, eps, stream));
+  }
+
+  // times every launch configuration on stage 1 (q -> q1; q is not
+  // changed) and keeps the fastest
+  int autotune(double dt, int reps, double *us) {
+    MXM_HIP(hipStreamSynchronize(stream));
+    if (prec == MXM_PREC_FP64) {
+      const StageArgs<double> a = stage_args(0, dt, q.p, q1.p, q2.p);
+      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us);
+    } else {
+      const StageArgs<float> a = stage_args(0, dt, f.p, f1.p, f2.p);
+      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us);
+    }
+    launch = find_stage(dim, phys, variant, P, prec, cfg);
+    return cfg;
   }
 
   void step(double dt, int n) {
@@ -182,8 +204,9 @@ int create(int dim, const void *desc, int P, int phys, const double *prm_in, int
   s->NC = nc;
   s->prec = precision;
   if (prm_in) std::memcpy(s->prm, prm_in, sizeof(s->prm));
-  const int variant = phys_variant(phys, s->prm);
-  s->launch = find_stage(dim, phys, variant, P, precision);
+  s->variant = phys_variant(phys, s->prm);
+  s->cfg = stage_cfg_default(precision);
+  s->launch = find_stage(dim, phys, s->variant, P, precision, s->cfg);
   if (!s->launch) throw ArgError("no stage kernel for this (physics, flux, P)");
   RefElem re = build_refelem(dim, P);
   if (dim == 3) {
@@ -226,20 +249,27 @@ int mxm_get_constant(int which, int *value) {
     return MXM_OK;
   }
   if (which == 1) {
+    // stage kernels: (dimension, physics, variant, P, form, launch configuration)
     int n = 0;
-    for (int P = 1; P <= 5; ++P) {
-      // Euler variants: flux kind + 4 * entropy fix
-      for (int v = 0; v < 8; ++v)
-        n += (stage_3d_fp64(1, v, P) != nullptr) + (stage_2d_trait_fp64(1, v, P) != nullptr) +
-             (stage_3d_fp32(1, v, P) != nullptr) + (stage_2d_trait_fp32(1, v, P) != nullptr);
-      n += (stage_3d_fp64(0, 0, P) != nullptr) + (stage_2d_trait_fp64(0, 0, P) != nullptr) +
-           (stage_3d_fp32(0, 0, P) != nullptr) + (stage_2d_trait_fp32(0, 0, P) != nullptr);
-      for (int ph = 2; ph <= 6; ++ph)
-        for (int v = 0; v < 2; ++v)
-          if ((ph == 2 || ph == 3) || v == 0)
-            n += (stage_2d_module_fp64(ph, v, P) != nullptr) + (stage_2d_module_fp32(ph, v, P) != nullptr);
-    }
+    for (int prec : {MXM_PREC_FP64, MXM_PREC_FP32_MFMA})
+      for (int c = 0; c < stage_cfg_count(prec); ++c)
+        for (int P = 1; P <= 5; ++P)
+          for (int dim : {2, 3})
+            for (int ph = 0; ph <= 6; ++ph)
+              for (int v = 0; v < 8; ++v) {
+                // Euler variants: flux kind + 4 * entropy fix; Euler2D and SW: 2
+                const bool valid = ph == MXM_PHYS_EULER ? v != 4 : (ph == 2 || ph == 3) ? v < 2 : v == 0;
+                if (valid && find_stage(dim, ph, v, P, prec, c)) ++n;
+              }
     *value = n;
+    return MXM_OK;
+  }
+  if (which == 2) {
+    *value = stage_cfg_count(MXM_PREC_FP64);
+    return MXM_OK;
+  }
+  if (which == 3) {
+    *value = stage_cfg_count(MXM_PREC_FP32_MFMA);
     return MXM_OK;
   }
   return fail(MXM_ERR_ARG, "unknown constant");
@@ -501,6 +531,33 @@ int mxm_dg_time_stage_kernel(void *h, double dt, int nsteps, double *avg_us) {
     for (auto &e : ev) (void)hipEventDestroy(e);
     s->limiter = lim;
     *avg_us = (double)ms * 1e3 / (3.0 * nsteps);
+    return MXM_OK;
+  });
+}
+
+int mxm_dg_autotune(void *h, double dt, int reps, int *cfg, double *us, int cap) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    if (!(dt > 0.0) || reps <= 0) throw ArgError("bad autotune arguments");
+    double t[kStageCfgMax] = {0};
+    const int c = s->autotune(dt, reps, t);
+    if (cfg) *cfg = c;
+    if (us)
+      for (int k = 0; k < cap && k < stage_cfg_count(s->prec); ++k) us[k] = t[k];
+    return MXM_OK;
+  });
+}
+
+int mxm_dg_launch_config(void *h, int cfg, int *current) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    if (cfg >= 0) {
+      StageLaunch l = find_stage(s->dim, s->phys, s->variant, s->P, s->prec, cfg);
+      if (!l) throw ArgError("launch configuration out of range");
+      s->cfg = cfg;
+      s->launch = l;
+    }
+    if (current) *current = s->cfg;
     return MXM_OK;
   });
 }

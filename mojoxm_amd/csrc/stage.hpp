@@ -82,6 +82,27 @@ struct StageArgs {
 
 constexpr int kPrmMax = 16;  // MXM_PRM_MAX
 
+// Launch configurations of the stage kernel: each is an occupancy target
+// (waves per SIMD, i.e. 256-thread blocks per CU) that bounds the register
+// allocation; all configurations compute the same bits.  The autotuner
+// times them on the problem and keeps the fastest.  FP64 kernels need
+// 100-150 VGPRs, so only 5 waves is worth trying; the FP32 kernels fit
+// 5 or 6 waves, and the compiler's own choice (1: no bound) can be far off
+// (e.g. 250 VGPRs, 1 wave, for the 2D Euler kernels at P = 1).
+template <class R>
+struct StageCfg;
+template <>
+struct StageCfg<double> {
+  static constexpr int count = 2, dflt = 0;
+  static constexpr int waves[2] = {1, 5};
+};
+template <>
+struct StageCfg<float> {
+  static constexpr int count = 3, dflt = 1;
+  static constexpr int waves[3] = {1, 5, 6};
+};
+constexpr int kStageCfgMax = 3;
+
 // Elements per 256-thread block for NP nodes per element, rounded down to
 // whole lattice cells (2 triangles, 6 tets) when a cell fits: tiles are
 // blocks of cells, so the LDS is sized for what a tile can hold.  The host
@@ -170,8 +191,10 @@ __device__ __forceinline__ void load_item_trace(const StageArgs<R> &A, int e0, i
 
 typedef float mxm_float4 __attribute__((ext_vector_type(4)));
 
-template <class Phys, int ND, int P, class R>
-__global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A) {
+// W: occupancy target of the launch configuration, in waves per SIMD
+// (amdgpu_waves_per_eu); 1 leaves the register budget to the compiler.
+template <class Phys, int ND, int P, class R, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void dg_stage_kernel(StageArgs<R> A) {
   using S = Shape<ND, P>;
   constexpr bool kMFMA = std::is_same<R, float>::value;
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
@@ -455,13 +478,13 @@ __global__ __launch_bounds__(256) void dg_stage_kernel(StageArgs<R> A) {
   }
 }
 
-template <class Phys, int ND, int P, class R>
+template <class Phys, int ND, int P, class R, int W>
 hipError_t launch_stage(const void *args, hipStream_t s) {
   using S = Shape<ND, P>;
   const StageArgs<R> &a = *static_cast<const StageArgs<R> *>(args);
   const int nb = a.ntiles;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R>), dim3(nb), dim3(S::BLOCK), 0, s, a);
+  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R, W>), dim3(nb), dim3(S::BLOCK), 0, s, a);
   return hipGetLastError();
 }
 

@@ -10,18 +10,18 @@ namespace mxm {
 
 // trait physics (Advection, Euler with each Riemann solver) on tetrahedra
 struct Family3D {
-  template <int P, class R>
+  template <int P, class R, int W>
   static StageLaunch pick(int phys, int variant) {
-    if (phys == 0) return &launch_stage<Advection<R>, 3, P, R>;
+    if (phys == 0) return &launch_stage<Advection<R>, 3, P, R, W>;
     if (phys == 1) {
       switch (variant) {
-        case 0: return &launch_stage<Euler<R, 0>, 3, P, R>;
-        case 1: return &launch_stage<Euler<R, 1>, 3, P, R>;
-        case 2: return &launch_stage<Euler<R, 2>, 3, P, R>;
-        case 3: return &launch_stage<Euler<R, 3>, 3, P, R>;
-        case 5: return &launch_stage<Euler<R, 1, 1>, 3, P, R>;
-        case 6: return &launch_stage<Euler<R, 2, 1>, 3, P, R>;
-        case 7: return &launch_stage<Euler<R, 3, 1>, 3, P, R>;
+        case 0: return &launch_stage<Euler<R, 0>, 3, P, R, W>;
+        case 1: return &launch_stage<Euler<R, 1>, 3, P, R, W>;
+        case 2: return &launch_stage<Euler<R, 2>, 3, P, R, W>;
+        case 3: return &launch_stage<Euler<R, 3>, 3, P, R, W>;
+        case 5: return &launch_stage<Euler<R, 1, 1>, 3, P, R, W>;
+        case 6: return &launch_stage<Euler<R, 2, 1>, 3, P, R, W>;
+        case 7: return &launch_stage<Euler<R, 3, 1>, 3, P, R, W>;
       }
     }
     return nullptr;
@@ -30,18 +30,18 @@ struct Family3D {
 
 // the same trait physics on triangles (nz = 0)
 struct Family2DTrait {
-  template <int P, class R>
+  template <int P, class R, int W>
   static StageLaunch pick(int phys, int variant) {
-    if (phys == 0) return &launch_stage<Advection<R>, 2, P, R>;
+    if (phys == 0) return &launch_stage<Advection<R>, 2, P, R, W>;
     if (phys == 1) {
       switch (variant) {
-        case 0: return &launch_stage<Euler<R, 0>, 2, P, R>;
-        case 1: return &launch_stage<Euler<R, 1>, 2, P, R>;
-        case 2: return &launch_stage<Euler<R, 2>, 2, P, R>;
-        case 3: return &launch_stage<Euler<R, 3>, 2, P, R>;
-        case 5: return &launch_stage<Euler<R, 1, 1>, 2, P, R>;
-        case 6: return &launch_stage<Euler<R, 2, 1>, 2, P, R>;
-        case 7: return &launch_stage<Euler<R, 3, 1>, 2, P, R>;
+        case 0: return &launch_stage<Euler<R, 0>, 2, P, R, W>;
+        case 1: return &launch_stage<Euler<R, 1>, 2, P, R, W>;
+        case 2: return &launch_stage<Euler<R, 2>, 2, P, R, W>;
+        case 3: return &launch_stage<Euler<R, 3>, 2, P, R, W>;
+        case 5: return &launch_stage<Euler<R, 1, 1>, 2, P, R, W>;
+        case 6: return &launch_stage<Euler<R, 2, 1>, 2, P, R, W>;
+        case 7: return &launch_stage<Euler<R, 3, 1>, 2, P, R, W>;
       }
     }
     return nullptr;
@@ -51,37 +51,48 @@ struct Family2DTrait {
 // 2D module physics: Euler2D Rusanov/HLLC, SW Rusanov/HLL, GLM-MHD,
 // Maxwell, Advection2D
 struct Family2DModule {
-  template <int P, class R>
+  template <int P, class R, int W>
   static StageLaunch pick(int phys, int variant) {
     switch (phys) {
-      case 2: return variant ? &launch_stage<Euler2D<R, 1>, 2, P, R> : &launch_stage<Euler2D<R, 0>, 2, P, R>;
-      case 3: return variant ? &launch_stage<SW2D<R, 1>, 2, P, R> : &launch_stage<SW2D<R, 0>, 2, P, R>;
-      case 4: return &launch_stage<MHDGLM2D<R>, 2, P, R>;
-      case 5: return &launch_stage<Maxwell2D<R>, 2, P, R>;
-      case 6: return &launch_stage<Advection2D<R>, 2, P, R>;
+      case 2: return variant ? &launch_stage<Euler2D<R, 1>, 2, P, R, W> : &launch_stage<Euler2D<R, 0>, 2, P, R, W>;
+      case 3: return variant ? &launch_stage<SW2D<R, 1>, 2, P, R, W> : &launch_stage<SW2D<R, 0>, 2, P, R, W>;
+      case 4: return &launch_stage<MHDGLM2D<R>, 2, P, R, W>;
+      case 5: return &launch_stage<Maxwell2D<R>, 2, P, R, W>;
+      case 6: return &launch_stage<Advection2D<R>, 2, P, R, W>;
     }
     return nullptr;
   }
 };
 
+template <class F, int P, class R>
+StageLaunch pick_cfg(int phys, int variant, int cfg) {
+  switch (cfg) {
+    case 0: return F::template pick<P, R, StageCfg<R>::waves[0]>(phys, variant);
+    case 1: return F::template pick<P, R, StageCfg<R>::waves[1]>(phys, variant);
+  }
+  if constexpr (StageCfg<R>::count > 2)
+    if (cfg == 2) return F::template pick<P, R, StageCfg<R>::waves[2]>(phys, variant);
+  return nullptr;
+}
+
 template <class F, class R>
-StageLaunch pick_order(int phys, int variant, int P) {
+StageLaunch pick_order(int phys, int variant, int P, int cfg) {
   switch (P) {
-    case 1: return F::template pick<1, R>(phys, variant);
-    case 2: return F::template pick<2, R>(phys, variant);
-    case 3: return F::template pick<3, R>(phys, variant);
-    case 4: return F::template pick<4, R>(phys, variant);
-    case 5: return F::template pick<5, R>(phys, variant);
+    case 1: return pick_cfg<F, 1, R>(phys, variant, cfg);
+    case 2: return pick_cfg<F, 2, R>(phys, variant, cfg);
+    case 3: return pick_cfg<F, 3, R>(phys, variant, cfg);
+    case 4: return pick_cfg<F, 4, R>(phys, variant, cfg);
+    case 5: return pick_cfg<F, 5, R>(phys, variant, cfg);
   }
   return nullptr;
 }
 
 // one entry per (family, form), defined in the stage*.hip units
-StageLaunch stage_3d_fp64(int phys, int variant, int P);
-StageLaunch stage_3d_fp32(int phys, int variant, int P);
-StageLaunch stage_2d_trait_fp64(int phys, int variant, int P);
-StageLaunch stage_2d_trait_fp32(int phys, int variant, int P);
-StageLaunch stage_2d_module_fp64(int phys, int variant, int P);
-StageLaunch stage_2d_module_fp32(int phys, int variant, int P);
+StageLaunch stage_3d_fp64(int phys, int variant, int P, int cfg);
+StageLaunch stage_3d_fp32(int phys, int variant, int P, int cfg);
+StageLaunch stage_2d_trait_fp64(int phys, int variant, int P, int cfg);
+StageLaunch stage_2d_trait_fp32(int phys, int variant, int P, int cfg);
+StageLaunch stage_2d_module_fp64(int phys, int variant, int P, int cfg);
+StageLaunch stage_2d_module_fp32(int phys, int variant, int P, int cfg);
 
 }  // namespace mxm

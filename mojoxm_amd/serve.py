@@ -91,10 +91,11 @@ def serve(requests, precision=PREC_FP64, budget=64 << 30, chunk=16, keep_results
     return ServeReport(len(reqs), makespan, [lat[r.index] for r in reqs], peak, budget, out)
 
 
-def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False):
+def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False, tune=False):
     """Each request alone with its own Solver, in arrival order.  Returns
     (solve seconds per request, results); time-to-solution of a FIFO server
-    follows from the arrivals: finish_i = max(arrival_i, finish_{i-1}) + t_i."""
+    follows from the arrivals: finish_i = max(arrival_i, finish_{i-1}) + t_i.
+    tune: autotune each solver's launch configuration first (untimed)."""
     q0 = states or {r.index: initial_state(r) for r in requests}
     secs, out = {}, {}
     for r in sorted(requests, key=lambda r: r.arrival):
@@ -102,6 +103,8 @@ def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False
             s.set_state(q0[r.index])
             if r.limiter:
                 s.enable_cell_limiter(True, r.eps)
+            if tune:
+                s.autotune(r.dt, 5)
             s.synchronize()
             t = time.perf_counter()
             s.step_ssprk3(r.dt, r.nsteps)

@@ -16,6 +16,13 @@ PREC_FP64 = 0       # plain FP64 (include/mxm_dg.h MXM_PREC_FP64)
 PREC_FP32_MFMA = 1  # FP32 state and fluxes, volume + LIFT on FP32 MFMA (MXM_PREC_FP32_MFMA)
 
 
+def launch_config_count(precision: int) -> int:
+    """Launch configurations of the stage kernels of one arithmetic form."""
+    v = C.c_int()
+    L.check(L.lib().mxm_get_constant(2 if precision == PREC_FP64 else 3, C.byref(v)), "mxm_get_constant")
+    return v.value
+
+
 class Solver:
     """DG solver on a periodic Kuhn-tet (ndim=3) or /-split triangle
     (ndim=2) mesh.
@@ -125,6 +132,25 @@ class Solver:
         m = L.MemReport()
         L.check(L.lib().mxm_dg_memory_report(self._h, C.byref(m)), "mxm_dg_memory_report")
         return MemoryReport(m.rk_stage, m.dg_operators, m.limiter, m.mesh_connectivity, m.halo_device, m.halo_pinned)
+
+    def autotune(self, dt: float, reps: int = 20):
+        """Time every launch configuration of the stage kernel on this mesh
+        (stage 1 of SSPRK3, which leaves the state unchanged) and keep the
+        fastest.  Returns (chosen index, microseconds per configuration)."""
+        us = (C.c_double * 4)()
+        cfg = C.c_int()
+        L.check(L.lib().mxm_dg_autotune(self._h, dt, reps, C.byref(cfg), us, 4), "mxm_dg_autotune")
+        return cfg.value, [us[k] for k in range(launch_config_count(self.precision))]
+
+    @property
+    def launch_config(self) -> int:
+        v = C.c_int()
+        L.check(L.lib().mxm_dg_launch_config(self._h, -1, C.byref(v)), "mxm_dg_launch_config")
+        return v.value
+
+    @launch_config.setter
+    def launch_config(self, cfg: int) -> None:
+        L.check(L.lib().mxm_dg_launch_config(self._h, int(cfg), None), "mxm_dg_launch_config")
 
     def time_stage_kernel(self, dt: float, nsteps: int) -> float:
         """Average stage-kernel duration in microseconds (HIP events on the
