@@ -179,10 +179,12 @@ __device__ __forceinline__ void load_item_records(const StageArgs<R> &A, int til
   it.pl = jr.y - e0;
 }
 
-template <int NP, class R, int NC>
-__device__ __forceinline__ void load_item_trace(const StageArgs<R> &A, int e0, int nE, FaceItem<R, NC> &it) {
+// A partner slot in [0, EPB) is in the tile: partners are real elements,
+// which occupy the tile's first slots.
+template <int NP, int EPB, class R, int NC>
+__device__ __forceinline__ void load_item_trace(const StageArgs<R> &A, int e0, FaceItem<R, NC> &it) {
   const bool need = it.packed >= 0 && !(it.side == 0 && it.bc != 0);
-  if (need && !(it.pl >= 0 && it.pl < nE)) {
+  if (need && !(it.pl >= 0 && it.pl < EPB)) {
     const R *g = A.q_in + ((size_t)(e0 + it.pl) * NP + it.pn) * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) it.qp[c] = g[c];
@@ -214,57 +216,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   const int inst = A.tile_inst ? A.tile_inst[tile] : 0;
   if (A.inst_active && !A.inst_active[inst]) return;
   const int e0 = tile * EPB;
-  const int nE = A.tile_count[tile];
-  if (nE <= 0) return;
   const Phys phys = Phys::from(A.inst_prm + (size_t)inst * kPrmMax);
   const double cdt = A.c * (A.inst_dt ? A.inst_dt[inst] : A.dt);
   const double post = A.inst_post ? A.inst_post[inst] : A.post_scale;
   const int nitems = A.maxj * NFP;  // padding items have packed < 0
-  // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
-  const bool node = tid < nE * NP;
 
   // ---- 1. loads --------------------------------------------------------
   // First everything addressed from the tile index -- the q_in tile, the
   // geometry, the first face item's records and q_a of the own node --
   // then the partner trace that waits for the records; the tile and the
-  // geometry go to LDS last.
+  // geometry go to LDS last.  Every tile owns EPB element slots in every
+  // per-slot array (padding slots hold zeros), so these loads cover the
+  // whole tile and need not wait for the tile's element count.
   constexpr int QPT = (TILE + S::BLOCK - 1) / S::BLOCK;  // tile values per thread
   constexpr int NJ = EPB * ND * ND, NG = NJ + EPB;      // invJ, then inv_vol
   constexpr int GPT = (NG + S::BLOCK - 1) / S::BLOCK;
-  const int nq = nE * NP * NC;
   R tq[QPT];
   {
     const R *src = A.q_in + (size_t)e0 * NP * NC;
 #pragma unroll
     for (int k = 0; k < QPT; ++k)
-      if (k * S::BLOCK + tid < nq) tq[k] = src[k * S::BLOCK + tid];
+      if (k * S::BLOCK + tid < TILE) tq[k] = src[k * S::BLOCK + tid];
   }
   double tg[GPT];
 #pragma unroll
   for (int k = 0; k < GPT; ++k) {
     const int t = k * S::BLOCK + tid;
-    if (t < nE * ND * ND)
+    if (t < NJ)
       tg[k] = A.invJ[(size_t)e0 * ND * ND + t];
-    else if (t >= NJ && t < NJ + nE)
+    else if (t < NG)
       tg[k] = A.inv_vol[e0 + t - NJ];
   }
   const int fm = tid < NF * NFP ? A.fmap[tid] : 0;
   FaceItem<R, NC> it;
   if (tid < nitems) load_item_records<NFP>(A, tile, tid, e0, it);
   R qa_in[NC];  // q_a of the own node, in the state precision until 2b
-  if (node) {
+  if (tid < EPB * NP) {
     const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) qa_in[c] = g[c];
   }
-  if (tid < nitems) load_item_trace<NP>(A, e0, nE, it);
+  const int nE = A.tile_count[tile];
+  // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
+  const bool node = tid < nE * NP;
+  if (tid < nitems) load_item_trace<NP, EPB>(A, e0, it);
 #pragma unroll
   for (int k = 0; k < QPT; ++k)
-    if (k * S::BLOCK + tid < nq) sW[k * S::BLOCK + tid] = tq[k];
+    if (k * S::BLOCK + tid < TILE) sW[k * S::BLOCK + tid] = tq[k];
 #pragma unroll
   for (int k = 0; k < GPT; ++k) {
     const int t = k * S::BLOCK + tid;
-    if (t < nE * ND * ND || (t >= NJ && t < NJ + nE)) sJ[t] = tg[k];
+    if (t < NG) sJ[t] = tg[k];
   }
   if (tid < NF * NFP) sMap[tid] = fm;
   __syncthreads();
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   for (int t = tid; t < nitems; t += S::BLOCK) {
     if (t != tid) {
       load_item_records<NFP>(A, tile, t, e0, it);
-      load_item_trace<NP>(A, e0, nE, it);
+      load_item_trace<NP, EPB>(A, e0, it);
     }
     if (it.packed < 0) continue;
     const int el = it.packed & 127, lf = (it.packed >> 7) & 3, pin = (it.packed >> 9) & 127;
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     const int mine = (el * NP + sMap[lf * NFP + it.r]) * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) qm[c] = sQ[mine + c];
-    if (it.pl >= 0 && it.pl < nE) {
+    if (it.pl >= 0 && it.pl < EPB) {
       const int other = (it.pl * NP + it.pn) * NC;
 #pragma unroll
       for (int c = 0; c < NC; ++c) qo[c] = sQ[other + c];
