@@ -137,6 +137,55 @@ def batched_leg(M, args):
             "l2_fp32_vs_fp64_sample_max": float(np.max(err64))}
 
 
+def taylor_green_leg(M, args):
+    """C3: compressible Taylor-Green vortex, 32^3 cubes x 6 tets, P=3,
+    Rusanov, 200 SSPRK3 steps, FP64 against FP32-MFMA from the same
+    FP32-representable initial state; KE and enstrophy tracked every 20
+    steps with the device's compensated FP64 diagnostics."""
+    pb = M.problems.taylor_green_3d()
+    mesh = M.LocalMesh(*pb.n, *pb.L, P=pb.P)
+    q0 = pb.ic(mesh.elem_node_xyz).astype(np.float32).astype(np.float64)
+    chunk = 20
+    res, finals = {}, {}
+    for name, prec, word in (("fp64", M.PREC_FP64, 8), ("fp32_mfma", M.PREC_FP32_MFMA, 4)):
+        with M.Solver(pb.n, pb.L, pb.physics, P=pb.P, precision=prec) as s:
+            U = s.dof_count()
+            s.set_state(q0)
+            cfg, cfg_us = s.autotune(pb.dt, 10)
+            s.set_state(q0)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step_ssprk3(pb.dt, pb.nsteps)
+            s.synchronize()
+            wall = time.perf_counter() - t0
+            # the same run again, stopping every `chunk` steps for diagnostics
+            s.set_state(q0)
+            d = s.diagnostics()
+            hist = [(0, d.kinetic_energy, d.enstrophy)]
+            for k in range(chunk, pb.nsteps + 1, chunk):
+                s.step_ssprk3(pb.dt, chunk)
+                d = s.diagnostics()
+                hist.append((k, d.kinetic_energy, d.enstrophy))
+            finals[name] = s.get_state()
+            s.set_state(q0)
+            avg_us = s.time_stage_kernel(pb.dt, 20)
+        alg = 8.0 / 3.0 * U * word
+        res[name] = {"value": 3.0 * U * pb.nsteps / wall, "ms_per_step": wall / pb.nsteps * 1e3,
+                     "kernel_avg_us": avg_us, "achieved_gbs": alg / (avg_us * 1e-6) / 1e9,
+                     "frac": alg / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                     "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg]},
+                     "ke_enstrophy_every_20_steps": [[k, ke, en] for k, ke, en in hist]}
+    h64 = res["fp64"]["ke_enstrophy_every_20_steps"]
+    h32 = res["fp32_mfma"]["ke_enstrophy_every_20_steps"]
+    return {"workload": "taylor_green_3d: 32^3 x 6 tets, P=3, NC=5, Rusanov, Ma 0.3, 200 steps",
+            "dof_per_stage": U, "dt": pb.dt, "fp64": res["fp64"], "fp32_mfma": res["fp32_mfma"],
+            "speedup_fp32_mfma_vs_fp64": res["fp32_mfma"]["value"] / res["fp64"]["value"],
+            "l2_fp32_vs_fp64_final": rel_l2(finals["fp32_mfma"], finals["fp64"]),
+            "ke_rel_diff_final": abs(h32[-1][1] / h64[-1][1] - 1.0),
+            "enstrophy_rel_diff_final": abs(h32[-1][2] / h64[-1][2] - 1.0),
+            "mfma_and_traffic": "profiles/dg_stage_kernel_tgv_3d_p3.md"}
+
+
 def serving_leg(M, args):
     """Seeded Poisson stream (dam break / Sod / PEC cavity, n in [16,256],
     P in {2,3,4}, T in [0.1,1]) under continuous batching (FP32-MFMA) with
@@ -173,6 +222,7 @@ def main() -> int:
     ap.add_argument("--no-accuracy", action="store_true", help="skip the full run to T=2 (for profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
+    ap.add_argument("--no-tgv", action="store_true", help="skip the 3D Taylor-Green leg")
     ap.add_argument("--ensemble", type=int, default=512, help="instances of the batched ensemble")
     ap.add_argument("--solo-sample", type=int, default=32, help="ensemble instances also solved one at a time")
     ap.add_argument("--requests", type=int, default=16, help="requests of the serving stream")
@@ -267,6 +317,7 @@ def main() -> int:
     for _, solver in res.values():
         solver.close()
 
+    tgv = taylor_green_leg(M, args) if not args.no_tgv else None
     batched = batched_leg(M, args) if not args.no_batch else None
     serving = serving_leg(M, args) if not args.no_serve else None
 
@@ -295,6 +346,7 @@ def main() -> int:
             "fp64": fp64,
             "speedup_fp32_mfma_vs_fp64": r32["value"] / r64["value"],
         },
+        "taylor_green": tgv,
         "batched": batched,
         "serving": serving,
         "roofline": r32["roofline"],

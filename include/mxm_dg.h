@@ -143,6 +143,11 @@ int mxm_refelem_3d(int P, double *D, double *Lift, double *w, int *f2e);
 /* ReferenceElement2D[P], src/reference_2d.mojo:130-261: D[2*NP*NP],
  * Lift[3*NP*(P+1)], w[NP], e2e[3*(P+1)]. */
 int mxm_refelem_2d(int P, double *D, double *Lift, double *w, int *e2e);
+/* Nodal differentiation matrices of the same basis, Dn[dim][NP][NP] with
+ * Dn[k][i][j] = d phi_j / d r_k at node i (host only; the diagnostics'
+ * vorticity uses them).  The reference has no such matrix: its volume
+ * operator is the weak D_ref = M^-1 S (src/reference.mojo:605-620). */
+int mxm_refelem_deriv(int dim, int P, double *Dn);
 /* LocalMesh[P] (src/local_mesh.mojo:1241-1485) / LocalMesh2D[P]
  * (src/local_mesh_2d.mojo:103-395) built on the host. */
 int mxm_mesh_create_3d(const mxm_mesh3d_desc *desc, int P, void **mesh);
@@ -187,6 +192,16 @@ int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
  * own stream around them; advances the state.  No reference counterpart:
  * it is the measurement hook that the roofline figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
+/* Domain diagnostics of the current state, computed on the device:
+ * out[0..NC) = int q_c dV, out[NC..2NC) = int q_c^2 dV, both mass-weighted
+ * (V_e sum_i w_i f(q_i) with the node weights), out[2NC..3NC) = max |q_c|,
+ * out[3NC] = kinetic energy int 1/2 rho |u|^2 dV, out[3NC+1] = enstrophy
+ * int 1/2 rho |curl u|^2 dV (NaN for physics without a velocity); sums in
+ * compensated FP64, in a fixed order.  *n = 3*NC + 2; cap must be >= *n.
+ * Replaces DiagnosticsWriter.record's per-component download and host sum
+ * (src/diagnostics.mojo:161-223), whose unweighted nodal mean (:170) is
+ * replaced by the mass-weighted integral. */
+int mxm_dg_diagnostics(void *solver, double *out, int cap, int *n);
 /* Launch configurations (no reference counterpart; the reference has one
  * launch shape per kernel).  Each configuration of the stage kernel is an
  * occupancy target (waves per SIMD) that bounds its register allocation;
@@ -240,6 +255,8 @@ int mxm_batch_poll(void *batch, int *ids, int cap, int *count);
 int mxm_batch_status(void *batch, int id, int *steps_done);
 /* State of one instance, reference order [K*NP*NC], FP64. */
 int mxm_batch_get_state(void *batch, int id, double *q, long long n);
+/* mxm_dg_diagnostics of one instance. */
+int mxm_batch_diagnostics(void *batch, int id, double *out, int cap, int *n);
 /* Drop an instance; its device memory is reclaimed at the next step. */
 int mxm_batch_remove(void *batch, int id);
 /* Device bytes held by the batch. */

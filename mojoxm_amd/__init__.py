@@ -2,8 +2,9 @@
 side and SSPRK3 time step: hand-written HIP stage kernels behind the C ABI
 of libmxm_hip.so (include/mxm_dg.h), with this package as the host-side
 mirror of the reference's Solver / driver interface."""
-from . import batch, physics, problems, serve
+from . import batch, diagnostics, mxmq, physics, problems, serve
 from ._lib import MxmError
+from .diagnostics import DiagComponents, Diagnostics, DiagnosticsWriter
 from .boundary import (BC_INFLOW, BC_INTERIOR, BC_OUTFLOW, BC_WALL, BoundaryConditions,
                        BoundaryConditions2D)
 from .memory_report import MemoryReport, ThroughputReport
@@ -14,6 +15,7 @@ from .ssprk3 import SSPRK3StagePlan, ssprk3_stage_plans
 from .time_integrator import TimeLoopResult, apriori_dt, fixed_steps, run_ssprk3_loop
 
 __all__ = [
+    "batch", "diagnostics", "mxmq", "serve", "DiagComponents", "Diagnostics", "DiagnosticsWriter",
     "physics", "problems", "MxmError", "BC_INFLOW", "BC_INTERIOR", "BC_OUTFLOW", "BC_WALL",
     "BoundaryConditions", "BoundaryConditions2D", "MemoryReport", "ThroughputReport", "LocalMesh", "LocalMesh2D",
     "ReferenceElement", "ReferenceElement2D", "num_tet_nodes", "num_tri_nodes", "Solver", "SSPRK3StagePlan",

@@ -49,6 +49,7 @@ EXPORTS = {
     "mxm_get_constant": ([C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_refelem_3d": ([C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mxm_refelem_2d": ([C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mxm_refelem_deriv": ([C.c_int, C.c_int, C.c_void_p], C.c_int),
     "mxm_mesh_create_3d": ([C.POINTER(Mesh3DDesc), C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mxm_mesh_create_2d": ([C.POINTER(Mesh2DDesc), C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mxm_mesh_info": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
@@ -72,9 +73,11 @@ EXPORTS = {
     "mxm_dg_autotune": ([C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
                         C.c_int),
     "mxm_dg_launch_config": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mxm_dg_diagnostics": ([C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_destroy": ([C.c_void_p], C.c_int),
     "mxm_batch_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mxm_batch_autotune": ([C.c_void_p, C.c_int], C.c_int),
+    "mxm_batch_diagnostics": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_add_2d": ([C.c_void_p, C.POINTER(Mesh2DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,
                           C.c_int, C.c_double, C.c_void_p, C.c_longlong, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_add_3d": ([C.c_void_p, C.POINTER(Mesh3DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,

@@ -108,6 +108,15 @@ class Batch:
         L.check(L.lib().mxm_batch_get_state(self._h, ident, q.ctypes.data, q.size), "mxm_batch_get_state")
         return q
 
+    def diagnostics(self, ident: int):
+        """mxm_dg_diagnostics of one instance."""
+        from .diagnostics import Diagnostics
+        nc = self._shape[ident][2]
+        out = (C.c_double * (3 * nc + 2))()
+        n = C.c_int()
+        L.check(L.lib().mxm_batch_diagnostics(self._h, ident, out, 3 * nc + 2, C.byref(n)), "mxm_batch_diagnostics")
+        return Diagnostics.from_flat(list(out), nc)
+
     def remove(self, ident: int) -> None:
         L.check(L.lib().mxm_batch_remove(self._h, ident), "mxm_batch_remove")
         self._shape.pop(ident, None)

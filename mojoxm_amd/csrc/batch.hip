@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/mxm_dg.h"
+#include "diag.hpp"
 #include "limiter.hpp"
 #include "runtime.hpp"
 
@@ -70,6 +71,7 @@ struct BGroup {
   DevBuf<double> qd[3];
   DevBuf<float> qf[3];
   DevBuf<double> mean, invJ, inv_vol, jgeo, Dt, Lt, w, iprm, idt, ipost;
+  DevBuf<double> Dn, dpart, dout;  // diagnostics
   DevBuf<float> Aop;
   DevBuf<int4> conn, jobs;
   DevBuf<unsigned short> ipr;
@@ -88,7 +90,7 @@ struct BGroup {
   size_t bytes() const {
     size_t b = 0;
     for (int k = 0; k < 3; ++k) b += qd[k].bytes() + qf[k].bytes();
-    return b + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + Dt.bytes() + Lt.bytes() + w.bytes() +
+    return b + Dn.bytes() + dpart.bytes() + dout.bytes() + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + Dt.bytes() + Lt.bytes() + w.bytes() +
            iprm.bytes() + idt.bytes() + ipost.bytes() + Aop.bytes() + conn.bytes() + jobs.bytes() + ipr.bytes() +
            fmap.bytes() + tile_count.bytes() + tile_inst.bytes() + iactive.bytes() + isteps.bytes() +
            insteps.bytes();
@@ -158,6 +160,7 @@ struct Batch {
       g->Aop.upload(o.Aop.data(), o.Aop.size());
     }
     g->w.upload(re.w.data(), re.w.size());
+    g->Dn.upload(re.Dn.data(), re.Dn.size());
     g->fmap.upload(re.fmap.data(), re.fmap.size());
     return *groups.emplace(key, std::move(g)).first->second;
   }
@@ -460,6 +463,36 @@ struct Batch {
       read_state(g, g.qf, in, h);
   }
 
+  // diagnostics of one instance (Solver::diagnostics over its tiles)
+  void diagnostics(int id, double *h) {
+    BInst &in = inst(id);
+    BGroup &g = group_with(id);
+    if (g.dirty) repack(g);
+    const int nt = in.t.til.ntiles, NC = g.NC;
+    const size_t per = (size_t)kDiagSums * 2 + kDiagMaxNC;
+    if (g.dpart.n < (size_t)nt * per) g.dpart.alloc((size_t)nt * per);
+    if (g.dout.n < (size_t)(3 * NC + 2)) g.dout.alloc(3 * NC + 2);
+    auto fill = [&](auto &a) {
+      a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p, a.w = g.w.p, a.Dn = g.Dn.p, a.tile_count = g.tile_count.p;
+      a.tile0 = in.base_tile, a.ntiles = nt, a.NC = NC, a.nvel = diag_nvel(g.phys);
+      a.part = g.dpart.p, a.pmax = g.dpart.p + (size_t)nt * kDiagSums * 2, a.out = g.dout.p;
+    };
+    if (g.prec == MXM_PREC_FP64) {
+      DiagArgs<double> a{};
+      a.q = g.qd[0].p;
+      fill(a);
+      MXM_HIP(launch_diagnostics(g.dim, g.P, true, &a, stream));
+    } else {
+      DiagArgs<float> a{};
+      a.q = g.qf[0].p;
+      fill(a);
+      MXM_HIP(launch_diagnostics(g.dim, g.P, false, &a, stream));
+    }
+    MXM_HIP(hipMemcpyAsync(h, g.dout.p, (3 * NC + 2) * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MXM_HIP(hipStreamSynchronize(stream));
+    if (!diag_nvel(g.phys)) h[3 * NC] = h[3 * NC + 1] = std::nan("");
+  }
+
   void remove(int id) {
     BInst &in = inst(id);
     (void)in;
@@ -599,6 +632,17 @@ int mxm_batch_status(void *h, int id, int *steps_done) {
 int mxm_batch_get_state(void *h, int id, double *q, long long n) {
   return guarded([&] {
     as_batch(h)->get_state(id, q, n);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_diagnostics(void *h, int id, double *out, int cap, int *n) {
+  return guarded([&] {
+    Batch *b = as_batch(h);
+    const int nv = 3 * b->inst(id).NC + 2;
+    if (n) *n = nv;
+    if (!out || cap < nv) throw ArgError("diagnostics need 3*NC + 2 values");
+    b->diagnostics(id, out);
     return MXM_OK;
   });
 }

@@ -18,6 +18,7 @@ inline int tri_nodes(int P) { return (P + 1) * (P + 2) / 2; }
 struct RefElem {
   int dim = 0, P = 0, NP = 0, NFP = 0, NF = 0;
   std::vector<double> D, Lift, w;
+  std::vector<double> Dn;  // [dim][NP][NP] nodal differentiation: d phi_j / d r_k at node i
   std::vector<int> fmap;
   std::vector<int> bary;  // integer barycentric label of each node (sum = P)
 };

@@ -246,6 +246,23 @@ RefElem build_refelem(int dim, int P) {
     for (int i = 0; i < NP * NP; ++i) re.D[(size_t)d * NP * NP + i] = (double)S[i];
   }
 
+  // Nodal differentiation Dn[d][i][j] = d phi_j / d r_d (x_i): the
+  // derivative of each monomial at each node, through the coefficients.
+  re.Dn.assign((size_t)dim * NP * NP, 0.0);
+  for (int d = 0; d < dim; ++d)
+    for (int i = 0; i < NP; ++i)
+      for (int j = 0; j < NP; ++j) {
+        ld s = 0;
+        for (int k = 0; k < nm; ++k) {
+          if (mons[k][d] == 0) continue;
+          ld v = mons[k][d];
+          for (int dd = 0; dd < dim; ++dd)
+            for (int r = 0; r < mons[k][dd] - (dd == d ? 1 : 0); ++r) v *= X[i][dd];
+          s += C[j * nm + k] * v;
+        }
+        re.Dn[((size_t)d * NP + i) * NP + j] = (double)s;
+      }
+
   // Node weights w_i = int phi_i / |ref|.
   {
     const ld vol = dim == 3 ? (ld)1 / 6 : (ld)1 / 2;

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/mxm_dg.h"
+#include "diag.hpp"
 #include "host.hpp"
 #include "limiter.hpp"
 #include "runtime.hpp"
@@ -35,6 +36,7 @@ struct Solver {
   DevBuf<float> f, f1, f2;
   DevBuf<double> mean;
   DevBuf<double> invJ, inv_vol, jgeo, Dt, Lt, w, dprm;
+  DevBuf<double> Dn, dpart, dout;  // diagnostics: nodal derivative, tile partials, results
   DevBuf<float> Aop;
   DevBuf<int4> conn;
   DevBuf<int> fmap, tile_count;
@@ -87,6 +89,7 @@ struct Solver {
       Aop.upload(o.Aop.data(), o.Aop.size());
     }
     w.upload(re.w.data(), re.w.size());
+    Dn.upload(re.Dn.data(), re.Dn.size());
     fmap.upload(re.fmap.data(), re.fmap.size());
     dprm.upload(prm, MXM_PRM_MAX);
     MXM_HIP(hipDeviceSynchronize());
@@ -138,6 +141,33 @@ struct Solver {
     }
     launch = find_stage(dim, phys, variant, P, prec, cfg);
     return cfg;
+  }
+
+  // int q_c, int q_c^2, max |q_c|, KE, enstrophy of the current state
+  // (3*NC + 2 values; NaN for KE and enstrophy without a velocity)
+  void diagnostics(double *h) {
+    const size_t per = (size_t)kDiagSums * 2 + kDiagMaxNC;
+    if (dpart.n < (size_t)ntiles * per) dpart.alloc((size_t)ntiles * per);
+    if (dout.n < (size_t)(3 * NC + 2)) dout.alloc(3 * NC + 2);
+    auto fill = [&](auto &a) {
+      a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.w = w.p, a.Dn = Dn.p, a.tile_count = tile_count.p;
+      a.tile0 = 0, a.ntiles = ntiles, a.NC = NC, a.nvel = diag_nvel(phys);
+      a.part = dpart.p, a.pmax = dpart.p + (size_t)ntiles * kDiagSums * 2, a.out = dout.p;
+    };
+    if (prec == MXM_PREC_FP64) {
+      DiagArgs<double> a{};
+      a.q = q.p;
+      fill(a);
+      MXM_HIP(launch_diagnostics(dim, P, true, &a, stream));
+    } else {
+      DiagArgs<float> a{};
+      a.q = f.p;
+      fill(a);
+      MXM_HIP(launch_diagnostics(dim, P, false, &a, stream));
+    }
+    MXM_HIP(hipMemcpyAsync(h, dout.p, (3 * NC + 2) * sizeof(double), hipMemcpyDeviceToHost, stream));
+    MXM_HIP(hipStreamSynchronize(stream));
+    if (!diag_nvel(phys)) h[3 * NC] = h[3 * NC + 1] = std::nan("");
   }
 
   void step(double dt, int n) {
@@ -295,6 +325,16 @@ int mxm_refelem_2d(int P, double *D, double *Lift, double *w, int *e2e) {
     if (Lift) std::memcpy(Lift, re.Lift.data(), re.Lift.size() * sizeof(double));
     if (w) std::memcpy(w, re.w.data(), re.w.size() * sizeof(double));
     if (e2e) std::memcpy(e2e, re.fmap.data(), re.fmap.size() * sizeof(int));
+    return MXM_OK;
+  });
+}
+
+int mxm_refelem_deriv(int dim, int P, double *Dn) {
+  return guarded([&] {
+    if ((dim != 2 && dim != 3) || P < 1 || P > 6) throw ArgError("dim must be 2 or 3 and P in 1..6");
+    if (!Dn) throw ArgError("null output");
+    RefElem re = build_refelem(dim, P);
+    std::memcpy(Dn, re.Dn.data(), re.Dn.size() * sizeof(double));
     return MXM_OK;
   });
 }
@@ -558,6 +598,17 @@ int mxm_dg_launch_config(void *h, int cfg, int *current) {
       s->launch = l;
     }
     if (current) *current = s->cfg;
+    return MXM_OK;
+  });
+}
+
+int mxm_dg_diagnostics(void *h, double *out, int cap, int *n) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    const int nv = 3 * s->NC + 2;
+    if (n) *n = nv;
+    if (!out || cap < nv) throw ArgError("diagnostics need 3*NC + 2 values");
+    s->diagnostics(out);
     return MXM_OK;
   });
 }

@@ -29,6 +29,9 @@ class ReferenceElement:
         self.face_to_elem = np.zeros((4, NFP), np.int32)
         check(lib().mxm_refelem_3d(P, self.D_ref.ctypes.data, self.Lift_ref.ctypes.data,
                                    self.node_weights.ctypes.data, self.face_to_elem.ctypes.data), "mxm_refelem_3d")
+        # nodal differentiation d phi_j / d r_k at node i (diagnostics)
+        self.D_nodal = np.zeros((3, NP, NP))
+        check(lib().mxm_refelem_deriv(3, P, self.D_nodal.ctypes.data), "mxm_refelem_deriv")
 
 
 class ReferenceElement2D:
@@ -44,3 +47,5 @@ class ReferenceElement2D:
         self.edge_to_elem = np.zeros((3, NE), np.int32)
         check(lib().mxm_refelem_2d(P, self.D_ref.ctypes.data, self.Lift_ref.ctypes.data,
                                    self.node_weights.ctypes.data, self.edge_to_elem.ctypes.data), "mxm_refelem_2d")
+        self.D_nodal = np.zeros((2, NP, NP))
+        check(lib().mxm_refelem_deriv(2, P, self.D_nodal.ctypes.data), "mxm_refelem_deriv")

@@ -133,6 +133,17 @@ class Solver:
         L.check(L.lib().mxm_dg_memory_report(self._h, C.byref(m)), "mxm_dg_memory_report")
         return MemoryReport(m.rk_stage, m.dg_operators, m.limiter, m.mesh_connectivity, m.halo_device, m.halo_pinned)
 
+    def diagnostics(self):
+        """Mass-weighted int q_c dV and int q_c^2 dV, max |q_c|, kinetic
+        energy and enstrophy of the current state, computed on the device
+        in compensated FP64 (mxm_dg_diagnostics)."""
+        from .diagnostics import Diagnostics
+        nc = self.NC
+        out = (C.c_double * (3 * nc + 2))()
+        n = C.c_int()
+        L.check(L.lib().mxm_dg_diagnostics(self._h, out, 3 * nc + 2, C.byref(n)), "mxm_dg_diagnostics")
+        return Diagnostics.from_flat(list(out), nc)
+
     def autotune(self, dt: float, reps: int = 20):
         """Time every launch configuration of the stage kernel on this mesh
         (stage 1 of SSPRK3, which leaves the state unchanged) and keep the

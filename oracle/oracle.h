@@ -116,6 +116,14 @@ int orc_limiter(const orc_mesh_t *m, const orc_ops_t *ops, int nc, double *q,
 int orc_step_ssprk3(const orc_mesh_t *m, const orc_ops_t *ops, int phys,
                     const double *prm, double *q, double dt, int nsteps,
                     int limiter, double eps);
+/* Nodal differentiation matrices Dn[k][i][j] = d phi_j / d r_k at node i
+ * (ndim*NP*NP). */
+int orc_nodal_deriv(int ndim, int P, double *Dn);
+/* Domain diagnostics of q[K][NP][NC] (diag.c): out[3*NC + 2] = mass-weighted
+ * int q_c dV, int q_c^2 dV, max |q_c|, then KE and enstrophy (NaN for
+ * physics without a velocity). */
+int orc_diagnostics(const orc_mesh_t *m, const double *w, const double *Dn, int phys, const double *q,
+                    double *out);
 /* OpenMP thread count used by the stage loops (0 = runtime default). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

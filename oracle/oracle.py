@@ -75,6 +75,8 @@ def lib():
         L.orc_limiter.argtypes = [C.POINTER(_Mesh), C.POINTER(_Ops), C.c_int, _dp, C.c_double]
         L.orc_step_ssprk3.argtypes = [C.POINTER(_Mesh), C.POINTER(_Ops), C.c_int, _dp, _dp, C.c_double,
                                       C.c_int, C.c_int, C.c_double]
+        L.orc_nodal_deriv.argtypes = [C.c_int, C.c_int, _dp]
+        L.orc_diagnostics.argtypes = [C.POINTER(_Mesh), _dp, _dp, C.c_int, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -199,10 +201,31 @@ class Mesh:
             raise RuntimeError(f"orc_step_ssprk3 rc={rc}")
         return q
 
+    def diagnostics(self, phys, q):
+        """Mass-weighted int q_c dV, int q_c^2 dV, max |q_c|, KE, enstrophy
+        (diag.c); returns the flat array of 3*NC + 2 values."""
+        nc = lib().orc_phys_nc(phys)
+        if not hasattr(self, "_Dn"):
+            self._Dn = nodal_deriv(self.ndim, self.P).ravel()
+        out = np.zeros(3 * nc + 2)
+        rc = lib().orc_diagnostics(C.byref(self._s), self._w, self._Dn, phys, _f(q), out)
+        if rc:
+            raise RuntimeError(f"orc_diagnostics rc={rc}")
+        return out
+
     def limiter(self, nc, q, eps=0.1):
         q = np.array(q, dtype=np.float64, copy=True)
         lib().orc_limiter(C.byref(self._s), C.byref(self._o), nc, q.reshape(-1), eps)
         return q
+
+
+def nodal_deriv(ndim: int, P: int) -> np.ndarray:
+    """Dn[k][i][j] = d phi_j / d r_k at node i."""
+    NP = nt(P) if ndim == 3 else ntri(P)
+    Dn = np.zeros((ndim, NP, NP))
+    if lib().orc_nodal_deriv(ndim, P, Dn.reshape(-1)):
+        raise RuntimeError("orc_nodal_deriv failed")
+    return Dn
 
 
 def _f(a):

@@ -625,3 +625,54 @@ int orc_refelem2d(int P, double *D, double *Lift, double *w, int32_t *e2e,
     free(L);
     return 0;
 }
+
+/* Nodal differentiation: Dn[k][i][j] = d phi_j / d r_k at node i, from the
+ * same barycentric Lagrange basis (src/reference.mojo:346-427,
+ * src/reference_2d.mojo:87-128) by evaluating the derivative polynomials
+ * (poly_drdx) at the nodes.  The reference has no such matrix: its volume
+ * operator is the weak D_ref = M^-1 S; the diagnostics need the strong
+ * nodal derivative for the vorticity.  ndim = 2 or 3; Dn[ndim*NP*NP]. */
+static real poly_eval(const poly *p, const real *l) {
+    real s = 0.0;
+    for (int t = 0; t < p->n; ++t) {
+        const mono *m = &p->m[t];
+        real v = m->c;
+        for (int r = 0; r < m->a0; ++r) v *= l[0];
+        for (int r = 0; r < m->a1; ++r) v *= l[1];
+        for (int r = 0; r < m->a2; ++r) v *= l[2];
+        for (int r = 0; r < m->a3; ++r) v *= l[3];
+        s += v;
+    }
+    return s;
+}
+
+int orc_nodal_deriv(int ndim, int P, double *Dn) {
+    if ((ndim != 2 && ndim != 3) || P < 1 || P > 6) return -1;
+    int NP = ndim == 3 ? orc_num_tet_nodes(P) : orc_num_tri_nodes(P);
+    int *nodes = (int *)malloc(sizeof(int) * 4 * NP);
+    int *monos = (int *)malloc(sizeof(int) * 4 * NP);
+    if ((ndim == 3 ? tet_node_exponents(P, nodes) : tri_node_exponents(P, nodes)) != NP) return -2;
+    if (ndim == 3)
+        tet_monomials(P, monos);
+    else
+        tri_monomials(P, monos);
+    real *C = (real *)malloc(sizeof(real) * NP * NP);
+    if (basis_coefs(nodes, monos, NP, P, C)) return -3;
+    for (int j = 0; j < NP; ++j) {
+        poly phi = coefs_to_poly(C, j, monos, NP);
+        for (int k = 0; k < ndim; ++k) {
+            poly d = poly_drdx(&phi, k);
+            for (int i = 0; i < NP; ++i) {
+                real l[4];
+                for (int t = 0; t < 4; ++t) l[t] = (real)nodes[4 * i + t] / (real)P;
+                Dn[((size_t)k * NP + i) * NP + j] = (double)poly_eval(&d, l);
+            }
+            poly_free(&d);
+        }
+        poly_free(&phi);
+    }
+    free(nodes);
+    free(monos);
+    free(C);
+    return 0;
+}
