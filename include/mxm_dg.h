@@ -173,7 +173,9 @@ int mxm_dg_download_component(void *solver, int c, double *out, long long n);
 /* One Shu-Osher stage (0, 1, 2) of the plan in src/ssprk3.mojo:41-78,
  * followed by the limiter when it is enabled (src/solver.mojo:1027-1036). */
 int mxm_dg_rk_stage(void *solver, int stage, double dt);
-/* nsteps x step_ssprk3, src/solver.mojo:1071-1092 */
+/* nsteps x step_ssprk3, src/solver.mojo:1071-1092.  The steps are launched
+ * as replayed HIP graphs of 8 steps and of 1 step, rebuilt when dt, the
+ * limiter or the launch configuration changed since the last call. */
 int mxm_dg_step_ssprk3(void *solver, double dt, int nsteps);
 /* enable_cell_limiter, src/solver.mojo:735-751 (eps: Venkatakrishnan). */
 int mxm_dg_enable_limiter(void *solver, int on, double eps);
@@ -189,8 +191,11 @@ int mxm_dg_state_bytes_per_step(void *solver, long long *out);
 int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
 /* Average duration (microseconds) of the stage kernel over 3*nsteps
  * back-to-back launches (limiter off), from two HIP events on the solver's
- * own stream around them; advances the state.  No reference counterpart:
- * it is the measurement hook that the roofline figures are computed from. */
+ * own stream around nsteps SSPRK3 steps launched as mxm_dg_step_ssprk3
+ * launches them (replayed HIP graphs), so the average includes the
+ * launch-to-launch gaps; advances the state by nsteps + 9 steps.  No
+ * reference counterpart: it is the measurement hook that the roofline
+ * figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
 /* Domain diagnostics of the current state, computed on the device:
  * out[0..NC) = int q_c dV, out[NC..2NC) = int q_c^2 dV, both mass-weighted

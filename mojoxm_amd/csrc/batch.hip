@@ -76,16 +76,17 @@ struct BGroup {
   DevBuf<int4> conn, jobs;
   DevBuf<unsigned short> ipr;
   DevBuf<int> fmap, tile_count, tile_inst, iactive, isteps, insteps;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // replayed HIP graphs of kGraphSteps steps ([0]) and of one step ([1])
+  static constexpr int kGraphSteps = 8;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   GroupKey key{};
 
   ~BGroup() { drop_graph(); }
   void drop_graph() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
+    for (auto &e : exec) {
+      if (e) (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
   }
   size_t bytes() const {
     size_t b = 0;
@@ -406,30 +407,43 @@ struct Batch {
       launch_step(g, g.qf);
   }
 
-  // one SSPRK3 step of the group as a HIP graph (kernel nodes only)
-  void capture(BGroup &g) {
+  // kGraphSteps SSPRK3 steps, and one step, of the group as HIP graphs
+  // (kernel nodes only)
+  hipGraphExec_t capture_steps(BGroup &g, int n) {
     MXM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
-      launch_step(g);
+      for (int k = 0; k < n; ++k) launch_step(g);
     } catch (...) {
       hipGraph_t tmp = nullptr;
       (void)hipStreamEndCapture(stream, &tmp);
       if (tmp) (void)hipGraphDestroy(tmp);
       throw;
     }
-    MXM_HIP(hipStreamEndCapture(stream, &g.graph));
-    MXM_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+    hipGraph_t gr = nullptr;
+    MXM_HIP(hipStreamEndCapture(stream, &gr));
+    hipGraphExec_t e = nullptr;
+    const hipError_t rc = hipGraphInstantiate(&e, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    MXM_HIP(rc);
+    return e;
+  }
+  void capture(BGroup &g) {
+    g.exec[0] = capture_steps(g, BGroup::kGraphSteps);
+    g.exec[1] = capture_steps(g, 1);
   }
 
   void step(int n) {
     for (auto &kv : groups)
       if (kv.second->dirty) repack(*kv.second);
-    for (int k = 0; k < n; ++k)
+    auto launch_all = [&](int which) {
       for (auto &kv : groups) {
         BGroup &g = *kv.second;
         if (g.ntiles == 0) continue;
-        MXM_HIP(hipGraphLaunch(g.exec, stream));
+        MXM_HIP(hipGraphLaunch(g.exec[which], stream));
       }
+    };
+    for (; n >= BGroup::kGraphSteps; n -= BGroup::kGraphSteps) launch_all(0);
+    for (; n > 0; --n) launch_all(1);
   }
 
   void sync_counters() {

@@ -202,12 +202,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
   constexpr int TILE = EPB * NP * NC;  // values of one element tile
   constexpr int NCOL = EPB * NC;       // (element, component) columns of a tile
-  // sW holds ND tile-sized planes.  Plane 0 first holds the q_in tile as
-  // [el][j][c]; after the face phase the planes hold the contravariant
-  // nodal fluxes as [k][j][el][c]; after the contraction plane 0 holds
-  // (volume - lift) as [el][i][c].
-  __shared__ R sW[ND * TILE];
-  __shared__ R sS[NF * NFP * NCOL];           // [f*NFP + r][el][c]
+  // Rows of (element, component) columns are NCOLS apart: one word of
+  // padding, so that the node threads' row-strided writes of the flux
+  // planes spread over the LDS banks.
+  constexpr int NCOLS = NCOL + 1;
+  // sW holds ND flux planes of NP rows.  It first holds the q_in tile as
+  // [el][j][c] (TILE contiguous values); after the face phase the planes
+  // hold the contravariant nodal fluxes as [k][j][el][c] (row kj at
+  // kj*NCOLS); after the contraction it holds (volume - lift) as
+  // [el][i][c].
+  __shared__ R sW[ND * NP * NCOLS];
+  __shared__ R sS[NF * NFP * NCOLS];          // [f*NFP + r][el][c]
   __shared__ double sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
 
@@ -307,12 +312,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     // contraction; FP32 form: -inv_vol s area F*, ready for the product
     const R own = kMFMA ? R(-sJ[EPB * ND * ND + el] * sa) : R(sa);
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sS[(lf * NFP + it.r) * NCOL + el * NC + c] = own * Fs[c];
+    for (int c = 0; c < NC; ++c) sS[(lf * NFP + it.r) * NCOLS + el * NC + c] = own * Fs[c];
     if (pin) {
       const int plf = (it.packed >> 16) & 3, rs = it.rs;
       const R oth = kMFMA ? R(sJ[EPB * ND * ND + pin - 1] * sa) : R(-sa);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sS[(plf * NFP + rs) * NCOL + (pin - 1) * NC + c] = oth * Fs[c];
+      for (int c = 0; c < NC; ++c) sS[(plf * NFP + rs) * NCOLS + (pin - 1) * NC + c] = oth * Fs[c];
     }
   }
 
@@ -352,7 +357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 #pragma unroll
     for (int k = 0; k < ND; ++k)
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sW[k * TILE + (j * EPB + el) * NC + c] = Fk[k * NC + c];
+      for (int c = 0; c < NC; ++c) sW[(k * NP + j) * NCOLS + el * NC + c] = Fk[k * NC + c];
   }
   __syncthreads();
 
@@ -388,9 +393,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
           float bv = 0.f;
           if (col < NCOL) {
             if (k < KV)
-              bv = float(sW[k * NCOL + col]);
+              bv = float(sW[k * NCOLS + col]);
             else if (k < K)
-              bv = float(sS[(k - KV) * NCOL + col]);
+              bv = float(sS[(k - KV) * NCOLS + col]);
           }
 #pragma unroll
           for (int m = 0; m < MT; ++m) acc[m][w] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv, acc[m][w], 0, 0, 0);
@@ -434,7 +439,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
         const int el = p / NC;
 #pragma unroll 4
         for (int fr = 0; fr < NF * NFP; ++fr) {
-          const double sv = sS[fr * NCOL + p];
+          const double sv = sS[fr * NCOLS + p];
           const double *L = A.Lt + fr * NPAD + i0;
 #pragma unroll
           for (int ii = 0; ii < IH; ++ii) acc[m][ii] += L[ii] * sv;
@@ -444,7 +449,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
         for (int ii = 0; ii < IH; ++ii) acc[m][ii] *= niv;
 #pragma unroll 4
         for (int kj = 0; kj < ND * NP; ++kj) {
-          const double fv = sW[kj * NCOL + p];
+          const double fv = sW[kj * NCOLS + p];
           const double *D = A.Dt + kj * NPAD + i0;
 #pragma unroll
           for (int ii = 0; ii < IH; ++ii) acc[m][ii] += D[ii] * fv;
