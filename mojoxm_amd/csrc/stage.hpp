@@ -206,15 +206,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   // padding, so that the node threads' row-strided writes of the flux
   // planes spread over the LDS banks.
   constexpr int NCOLS = NCOL + 1;
-  // sW holds ND flux planes of NP rows.  It first holds the q_in tile as
-  // [el][j][c] (TILE contiguous values); after the face phase the planes
-  // hold the contravariant nodal fluxes as [k][j][el][c] (row kj at
-  // kj*NCOLS); after the contraction it holds (volume - lift) as
-  // [el][i][c].
-  __shared__ R sW[ND * NP * NCOLS];
-  __shared__ R sS[NF * NFP * NCOLS];          // [f*NFP + r][el][c]
+  // The contraction's right operand: KV = ND*NP flux-plane rows, then
+  // NF*NFP face-flux rows, one array.  FP32-MFMA: the product runs over
+  // KPAD rows in 4-deep steps and NT 16-column tiles; the array is long
+  // enough for every such read, and rows K..KPAD-1 hold zeros (their
+  // operator columns are zero; a column beyond NCOL only feeds an output
+  // column that is never stored).
+  constexpr int KV = ND * NP, K = KV + NF * NFP;
+  constexpr int KPAD = kMFMA ? mfma_kpad(ND, NP, NFP) : K;
+  constexpr int MT = mfma_mtiles(NP);
+  constexpr int NT = (NCOL + 15) / 16;
+  constexpr int NB = kMFMA && (KPAD - 1) * NCOLS + NT * 16 > K * NCOLS ? (KPAD - 1) * NCOLS + NT * 16 : K * NCOLS;
+  // FP32-MFMA: the operator [D | Lift] staged in LDS when it is at most
+  // 16 KB (2D, 3D P <= 3), else read from global memory (L2) in the loop
+  constexpr int NA = kMFMA ? MT * 16 * KPAD : 0;
+  constexpr bool kStageA = kMFMA && NA * 4 <= 16384;
+  // sW = rows 0..KV-1: first the q_in tile as [el][j][c] (TILE contiguous
+  // values); after the face phase the contravariant nodal fluxes as
+  // [k][j][el][c] (row kj at kj*NCOLS); after the contraction (volume -
+  // lift) as [el][i][c].  sS = rows KV..K-1: face fluxes [f*NFP + r][el][c].
+  __shared__ R sB[NB];
+  __shared__ float sA[kStageA ? NA : 1];
   __shared__ double sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
+  R *const sW = sB;
+  R *const sS = sB + KV * NCOLS;
 
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -253,6 +269,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
       tg[k] = A.inv_vol[e0 + t - NJ];
   }
   const int fm = tid < NF * NFP ? A.fmap[tid] : 0;
+  constexpr int APT = kStageA ? (NA + S::BLOCK - 1) / S::BLOCK : 0;
+  float ta[APT > 0 ? APT : 1];
+  if constexpr (kStageA) {
+#pragma unroll
+    for (int k = 0; k < APT; ++k)
+      if (k * S::BLOCK + tid < NA) ta[k] = A.Aop[k * S::BLOCK + tid];
+  }
   FaceItem<R, NC> it;
   if (tid < nitems) load_item_records<NFP>(A, tile, tid, e0, it);
   R qa_in[NC];  // q_a of the own node, in the state precision until 2b
@@ -274,6 +297,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     if (t < NG) sJ[t] = tg[k];
   }
   if (tid < NF * NFP) sMap[tid] = fm;
+  if constexpr (kStageA) {
+#pragma unroll
+    for (int k = 0; k < APT; ++k)
+      if (k * S::BLOCK + tid < NA) sA[k * S::BLOCK + tid] = ta[k];
+  }
+  if constexpr (kMFMA)
+    for (int t = K * NCOLS + tid; t < NB; t += S::BLOCK) sB[t] = R(0);
   __syncthreads();
   const R *sQ = sW;  // the q_in tile, [el][j][c], until the flux planes are written
 
@@ -365,13 +395,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr,
   // written into plane 0 as [el][i][c] after a barrier
   if constexpr (kMFMA) {
-    // Out[i][col] = sum_k Aop[i][k] In[k][col], In = [flux planes ; sS].
+    // Out[i][col] = sum_k Aop[i][k] B[k][col], B = [flux planes ; sS].
     // V_MFMA_F32_16X16X4F32 operands, lane l = 16 g + r: A[r][k0 + g],
     // B[k0 + g][col0 + r]; result rows 4 g + v (v < 4), column col0 + r.
-    constexpr int KV = ND * NP, K = KV + NF * NFP, KPAD = mfma_kpad(ND, NP, NFP);
-    constexpr int MT = mfma_mtiles(NP);
-    constexpr int NT = (NCOL + 15) / 16;  // 16-column tiles
-    constexpr int NTW = (NT + 3) / 4;     // tiles per wave
+    constexpr int NTW = (NT + 3) / 4;  // column tiles per wave
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int l = tid & 63, lr = l & 15, lg = l >> 4;
     mxm_float4 acc[MT][NTW];
@@ -379,28 +406,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     for (int m = 0; m < MT; ++m)
 #pragma unroll
       for (int w = 0; w < NTW; ++w) acc[m][w] = mxm_float4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-    for (int k0 = 0; k0 < KPAD; k0 += 4) {
-      const int k = k0 + lg;
+    auto kstep = [&](int k0) {
       float a[MT];
 #pragma unroll
-      for (int m = 0; m < MT; ++m) a[m] = A.Aop[(m * 16 + lr) * KPAD + k];
+      for (int m = 0; m < MT; ++m) {
+        const int ai = (m * 16 + lr) * KPAD + k0 + lg;
+        if constexpr (kStageA)
+          a[m] = sA[ai];
+        else
+          a[m] = A.Aop[ai];
+      }
 #pragma unroll
       for (int w = 0; w < NTW; ++w) {
         const int nt = wave * NTW + w;
         if (nt < NT) {
-          const int col = nt * 16 + lr;
-          float bv = 0.f;
-          if (col < NCOL) {
-            if (k < KV)
-              bv = float(sW[k * NCOLS + col]);
-            else if (k < K)
-              bv = float(sS[(k - KV) * NCOLS + col]);
-          }
+          const float bv = float(sB[(k0 + lg) * NCOLS + nt * 16 + lr]);
 #pragma unroll
           for (int m = 0; m < MT; ++m) acc[m][w] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv, acc[m][w], 0, 0, 0);
         }
       }
+    };
+    if constexpr (kStageA) {
+#pragma unroll
+      for (int k0 = 0; k0 < KPAD; k0 += 4) kstep(k0);
+    } else {
+#pragma unroll 4
+      for (int k0 = 0; k0 < KPAD; k0 += 4) kstep(k0);
     }
     __syncthreads();
 #pragma unroll
