@@ -47,6 +47,11 @@ extern "C" {
 #define MXM_PHYS_MHDGLM2D 4    /* src/local_mesh_2d_gpu_mhd_glm.mojo, NC=7 (2D)  */
 #define MXM_PHYS_MAXWELL2D 5   /* src/local_mesh_2d_gpu_maxwell.mojo, NC=6 (2D)  */
 #define MXM_PHYS_ADVECTION2D 6 /* src/local_mesh_2d_gpu_advection.mojo, NC=1 (2D)*/
+#define MXM_PHYS_SW 7          /* trait ShallowWater, src/shallow_water.mojo, NC=3 (3D) */
+#define MXM_PHYS_MAXWELL 8     /* trait Maxwell, src/maxwell.mojo, NC=6 (3D)     */
+#define MXM_PHYS_MHD 9         /* trait IdealMHD + GLM, src/mhd.mojo, NC=9 (3D)  */
+#define MXM_PHYS_TWOFLUID 10   /* trait FiveMomentTwoFluid, src/two_fluid.mojo,
+                                  NC=17 (3D)                                     */
 
 /* Euler flux kinds, src/euler.mojo:33-36 */
 #define MXM_FLUX_RUSANOV 0
@@ -66,8 +71,17 @@ extern "C" {
  *              [4] alpha_d [5..11] inflow (rho, rhou, rhov, Bx, By, E, psi)
  *  MAXWELL2D   [0] c [1..3] J [4..6] M [7..12] inflow (Ex, Ey, Ez, Bx, By, Bz)
  *  ADVECTION2D [0..1] v [2] inflow_q
+ *  SW          [0] g [1] h_min [2..4] inflow (h, hu, hv)  (src/shallow_water.mojo:31-52)
+ *  MAXWELL     [0] c [1..3] J [4..6] M [7..12] inflow (Ex, Ey, Ez, Bx, By, Bz)
+ *                                                         (src/maxwell.mojo:48-110)
+ *  MHD         [0] gamma [1] min_density [2] min_pressure [3] c_h [4] alpha_d
+ *              [5..13] inflow (rho, rhou, rhov, rhow, E, Bx, By, Bz, psi)
+ *                                                         (src/mhd.mojo:197-252)
+ *  TWOFLUID    [0] gamma_e [1] gamma_i [2] q_e [3] m_e [4] q_i [5] m_i [6] eps0
+ *              [7] c_light [8] c_h [9] alpha_d [10] min_density [11] min_pressure
+ *              [12..28] inflow (17 components)            (src/two_fluid.mojo:140-210)
  */
-#define MXM_PRM_MAX 16
+#define MXM_PRM_MAX 32
 
 /* Arithmetic of the stage kernels. */
 #define MXM_PREC_FP64 0      /* plain FP64 state, geometry and operators     */

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmxm_hip.so")
 
 MXM_OK = 0
-PRM_MAX = 16
+PRM_MAX = 32
 
 # mesh table ids (MXM_T_*)
 T_ELEM_NODE_XYZ, T_ELEM_INVJ, T_ELEM_INV_VOL, T_ELEM_FACES, T_ELEM_FACE_SIDE, T_ELEM_CANON_TO_REF, \

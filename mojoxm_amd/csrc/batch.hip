@@ -146,7 +146,7 @@ struct Batch {
     const RefElem &re = ref(in.dim, in.P);
     g->dim = in.dim, g->P = in.P, g->phys = in.phys, g->variant = in.variant, g->NC = in.NC, g->NP = re.NP,
     g->NFP = re.NFP, g->NF = re.NF;
-    g->epb = stage_epb(in.dim, re.NP);
+    g->epb = stage_epb(in.dim, re.NP, re.NFP, in.NC);
     g->prec = prec;
     g->limiter = in.limiter, g->eps = in.eps;
     g->key = key;
@@ -171,9 +171,10 @@ struct Batch {
     if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
     const int nc = phys_nc(phys);
     if (nc < 0) throw ArgError("unknown physics id");
-    if (dim == 3 && phys > MXM_PHYS_EULER) throw ArgError("2D module physics on a 3D mesh");
+    if (dim == 3 && phys > MXM_PHYS_EULER && phys < MXM_PHYS_SW) throw ArgError("2D module physics on a 3D mesh");
+    if (dim == 2 && phys >= MXM_PHYS_SW) throw ArgError("3D physics on a 2D mesh");
     if (!(dt > 0.0) || nsteps < 0) throw ArgError("bad time step or step count");
-    if (limiter && nc > kLimMaxNC) throw ArgError("limiter supports NC <= 8");
+    if (limiter && nc > kLimMaxNC) throw ArgError("limiter supports NC <= " + std::to_string(kLimMaxNC));
     auto in = std::make_unique<BInst>();
     in->dim = dim, in->P = P, in->phys = phys, in->NC = nc;
     if (prm) std::memcpy(in->prm, prm, sizeof(in->prm));
@@ -193,7 +194,7 @@ struct Batch {
     in->K = m.K, in->NP = m.NP;
     if (!q0 || n != (long long)m.K * m.NP * nc) throw ArgError("initial state length must be K*NP*NC");
     in->q0.assign(q0, q0 + n);
-    in->t = build_elem_tables(m, re);
+    in->t = build_elem_tables(m, re, nc);
     in->id = next_id++;
     BGroup &g = group_of(*in);
     g.members.push_back(in->id);
@@ -488,7 +489,7 @@ struct Batch {
     if (g.dout.n < (size_t)(3 * NC + 2)) g.dout.alloc(3 * NC + 2);
     auto fill = [&](auto &a) {
       a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p, a.w = g.w.p, a.Dn = g.Dn.p, a.tile_count = g.tile_count.p;
-      a.tile0 = in.base_tile, a.ntiles = nt, a.NC = NC, a.nvel = diag_nvel(g.phys);
+      a.epb = g.epb, a.tile0 = in.base_tile, a.ntiles = nt, a.NC = NC, a.nvel = diag_nvel(g.phys);
       a.part = g.dpart.p, a.pmax = g.dpart.p + (size_t)nt * kDiagSums * 2, a.out = g.dout.p;
     };
     if (g.prec == MXM_PREC_FP64) {
@@ -546,7 +547,7 @@ int footprint(int dim, const void *desc, int P, int phys, int prec, int limiter,
     check_bc(d->bc, 4);
     m = build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re);
   }
-  ElemTables t = build_elem_tables(m, re);
+  ElemTables t = build_elem_tables(m, re, nc);
   *out = (long long)instance_bytes(t, m.NP, nc, m.NFP, dim, prec, limiter != 0);
   return MXM_OK;
 }

@@ -59,10 +59,9 @@ __device__ __forceinline__ double wave_max(double m) {
 
 template <int ND, int P, class R>
 __global__ __launch_bounds__(256) void diag_tile_kernel(DiagArgs<R> A) {
-  using S = Shape<ND, P>;
-  constexpr int NP = S::NP, EPB = S::EPB;
+  constexpr int NP = Shape<ND, P>::NP;
   constexpr double kRefVol = ND == 3 ? 1.0 / 6.0 : 0.5;
-  __shared__ double sU[EPB * NP * 3];
+  __shared__ double sU[256 * 3];
   __shared__ double sPart[4][kDiagSums][2];
   __shared__ double sMax[4][kDiagMaxNC];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -71,7 +70,7 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(DiagArgs<R> A) {
   const int NC = A.NC;
   const bool node = tid < nE * NP;
   const int el = tid / NP, i = tid - el * NP;
-  const size_t slot = (size_t)T * EPB + el;
+  const size_t slot = (size_t)T * A.epb + el;
   double qv[kDiagMaxNC];
   double vw = 0.0;  // V_e w_i
 #pragma unroll

@@ -5,7 +5,7 @@
 
 namespace mxm {
 
-constexpr int kDiagMaxNC = 8;                   // components of any physics
+constexpr int kDiagMaxNC = 17;                  // components of any physics
 constexpr int kDiagSums = 2 * kDiagMaxNC + 2;   // per-tile sums: q_c, q_c^2, KE, enstrophy
 
 template <class R>
@@ -16,6 +16,7 @@ struct DiagArgs {
   const double *w;        // [NP] node weights, sum 1
   const double *Dn;       // [ND][NP][NP] nodal differentiation, d phi_j / d r_k at node i
   const int *tile_count;  // [tiles] elements per tile
+  int epb;                // element slots per tile (epb * NP <= 256)
   int tile0, ntiles;      // the tiles [tile0, tile0 + ntiles) of the arrays
   int NC;                 // components
   int nvel;               // velocity components after the density (0: no KE / enstrophy)
@@ -31,9 +32,11 @@ hipError_t launch_diagnostics(int dim, int P, bool fp64, const void *args, hipSt
 // velocity components stored after the density, per physics id (0: none)
 inline int diag_nvel(int phys) {
   switch (phys) {
-    case 1: return 3;  // trait Euler (rho, m[3], E), also on triangles
+    case 1:            // trait Euler (rho, m[3], E), also on triangles
+    case 9: return 3;  // 3D ideal MHD (rho, m[3], E, B[3], psi)
     case 2:            // Euler2D (rho, m[2], E)
     case 3:            // shallow water (h, h u[2])
+    case 7:            // 3D shallow water (h, h u[2])
     case 4: return 2;  // GLM-MHD (rho, m[2], B[2], E, psi)
   }
   return 0;

@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 
 namespace mxm {
-constexpr int kLimMaxNC = 8;
+constexpr int kLimMaxNC = 17;  // the widest state (two-fluid)
 // q: state [K*NP*NC] (double or float) limited in place; the means, the
 // bounds and theta are FP64 whatever the state precision.  mean: scratch
 // [K*NC]; w: node weights [NP]; conn: element-centric connectivity

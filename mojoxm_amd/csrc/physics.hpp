@@ -656,4 +656,290 @@ struct Advection2D {
   MXM_HD void limit(T *) const {}
 };
 
+// ---------------------------------------------------------------------------
+// The other trait physics of the reference's 3D solver (tets).  Each face
+// flux is Rusanov, 1/2 (F_n(qL) + F_n(qR)) - 1/2 alpha (qR - qL), the
+// boundary flux the same against the BC ghost.
+// ---------------------------------------------------------------------------
+
+template <class R, int NC>
+MXM_HD void rusanov_combine(const R *FL, const R *FR, const R *ql, const R *qr, R alpha, R *f) {
+  for (int c = 0; c < NC; ++c) f[c] = R(0.5) * (FL[c] + FR[c]) - R(0.5) * alpha * (qr[c] - ql[c]);
+}
+// v - 2 (v.n) n of the 3-vector at q + o (nd components take part)
+template <class R>
+MXM_HD void reflect3(const R *q, const R *n, int o, int nd, R *g) {
+  R vn = R(0);
+  for (int d = 0; d < nd; ++d) vn += q[o + d] * n[d];
+  for (int d = 0; d < nd; ++d) g[o + d] = q[o + d] - R(2) * vn * n[d];
+}
+
+// src/shallow_water.mojo:31-167: (h, hu, hv), the 2D system embedded in 3D
+template <class R>
+struct ShallowWater3 {
+  static constexpr int NC = 3;
+  static constexpr bool kSource = false, kLimit = false;
+  R g, hmin, inflow[3];
+  MXM_HD static ShallowWater3 from(const double *p) {
+    return {R(p[0]), R(p[1]), {R(p[2]), R(p[3]), R(p[4])}};
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    const R h = q[0] > hmin ? q[0] : hmin, ih = R(1) / h;
+    const R hu = q[1], hv = q[2], u = hu * ih, v = hv * ih, gh2 = R(0.5) * g * h * h;
+    F[0] = hu, F[1] = hu * u + gh2, F[2] = hu * v;
+    F[3] = hv, F[4] = hu * v, F[5] = hv * v + gh2;
+    if constexpr (ND == 3) F[6] = F[7] = F[8] = R(0);
+  }
+  MXM_HD R side(const R *q, R nx, R ny, R *F) const {
+    const R h = q[0] > hmin ? q[0] : hmin, ih = R(1) / h;
+    const R un = (q[1] * nx + q[2] * ny) * ih, gh2 = R(0.5) * g * h * h;
+    F[0] = q[1] * nx + q[2] * ny, F[1] = q[1] * un + gh2 * nx, F[2] = q[2] * un + gh2 * ny;
+    return rabs(un) + msqrt(g * h);
+  }
+  template <int ND = 3>
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R FL[3], FR[3];
+    const R a = rmax(side(ql, n[0], n[1], FL), side(qr, n[0], n[1], FR));
+    rusanov_combine<R, 3>(FL, FR, ql, qr, a, f);
+  }
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *gs) const {
+    for (int c = 0; c < 3; ++c) gs[c] = bc == BC_INFLOW ? inflow[c] : q[c];
+    if (bc == BC_WALL) reflect3(q, n, 1, 2, gs);
+  }
+  MXM_HD void source(const R *, R *s) const { s[0] = s[1] = s[2] = R(0); }
+  template <class T>
+  MXM_HD void limit(T *) const {}
+};
+
+// src/maxwell.mojo:48-255: (E, B), PEC walls, uniform J and M sources
+template <class R>
+struct Maxwell3 {
+  static constexpr int NC = 6;
+  static constexpr bool kSource = true, kLimit = false;
+  R c, J[3], M[3], inflow[6];
+  MXM_HD static Maxwell3 from(const double *p) {
+    Maxwell3 m;
+    m.c = R(p[0]);
+    for (int d = 0; d < 3; ++d) m.J[d] = R(p[1 + d]), m.M[d] = R(p[4 + d]);
+    for (int k = 0; k < 6; ++k) m.inflow[k] = R(p[7 + k]);
+    return m;
+  }
+  // (F.n)_E = c^2 (B x n), (F.n)_B = n x E
+  MXM_HD static void side(const R *q, R c2, const R *n, R *F) {
+    F[0] = c2 * (q[4] * n[2] - q[5] * n[1]);
+    F[1] = c2 * (q[5] * n[0] - q[3] * n[2]);
+    F[2] = c2 * (q[3] * n[1] - q[4] * n[0]);
+    F[3] = n[1] * q[2] - n[2] * q[1];
+    F[4] = n[2] * q[0] - n[0] * q[2];
+    F[5] = n[0] * q[1] - n[1] * q[0];
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    const R c2 = c * c;
+    for (int d = 0; d < ND; ++d) {
+      const R e[3] = {R(d == 0), R(d == 1), R(d == 2)};
+      side(q, c2, e, F + 6 * d);
+    }
+  }
+  template <int ND = 3>
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R FL[6], FR[6];
+    const R c2 = c * c;
+    side(ql, c2, n, FL);
+    side(qr, c2, n, FR);
+    rusanov_combine<R, 6>(FL, FR, ql, qr, c, f);
+  }
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    for (int k = 0; k < 6; ++k) g[k] = bc == BC_INFLOW ? inflow[k] : q[k];
+    if (bc == BC_WALL) {
+      const R En = q[0] * n[0] + q[1] * n[1] + q[2] * n[2];
+      for (int d = 0; d < 3; ++d) g[d] = R(2) * En * n[d] - q[d];
+      reflect3(q, n, 3, 3, g);
+    }
+  }
+  MXM_HD bool source_active() const { return true; }
+  MXM_HD void source(const R *, R *s) const {
+    const R c2 = c * c;
+    for (int d = 0; d < 3; ++d) s[d] = -(c2 * J[d]), s[3 + d] = -M[d];
+  }
+  template <class T>
+  MXM_HD void limit(T *) const {}
+};
+
+// src/mhd.mojo:46-367: (rho, rho u[3], E, B[3], psi), GLM cleaning with
+// the damping -alpha_d psi as a source
+template <class R>
+struct IdealMHD3 {
+  static constexpr int NC = 9;
+  static constexpr bool kSource = true, kLimit = false;
+  R gamma, rmin, pmin, ch, alpha, inflow[9];
+  MXM_HD static IdealMHD3 from(const double *p) {
+    IdealMHD3 m;
+    m.gamma = R(p[0]), m.rmin = R(p[1]), m.pmin = R(p[2]), m.ch = R(p[3]), m.alpha = R(p[4]);
+    for (int k = 0; k < 9; ++k) m.inflow[k] = R(p[5 + k]);
+    return m;
+  }
+  MXM_HD R pres(const R *q, R rho, R ir) const {
+    const R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * ir;
+    const R me = R(0.5) * (q[5] * q[5] + q[6] * q[6] + q[7] * q[7]);
+    const R p = (gamma - R(1)) * (q[4] - ke - me);
+    return p > pmin ? p : pmin;
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    const R rho = q[0] > rmin ? q[0] : rmin, ir = R(1) / rho;
+    const R u[3] = {q[1] * ir, q[2] * ir, q[3] * ir}, B[3] = {q[5], q[6], q[7]};
+    const R pt = pres(q, rho, ir) + R(0.5) * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    const R ub = u[0] * B[0] + u[1] * B[1] + u[2] * B[2], ch2 = ch * ch;
+    for (int d = 0; d < ND; ++d) {
+      R *Fd = F + 9 * d;
+      const R m = q[1 + d];
+      Fd[0] = m;
+      for (int k = 0; k < 3; ++k) Fd[1 + k] = m * u[k] - B[d] * B[k];
+      Fd[1 + d] += pt;
+      Fd[4] = u[d] * (q[4] + pt) - B[d] * ub;
+      for (int k = 0; k < 3; ++k) Fd[5 + k] = u[d] * B[k] - u[k] * B[d];
+      Fd[5 + d] = q[8];
+      Fd[8] = ch2 * B[d];
+    }
+  }
+  // normal flux and |u_n| + c_f of one side (src/mhd.mojo:76-89, 152-190)
+  MXM_HD R side(const R *q, const R *n, R *F) const {
+    const R rho = q[0] > rmin ? q[0] : rmin, ir = R(1) / rho;
+    const R u[3] = {q[1] * ir, q[2] * ir, q[3] * ir}, B[3] = {q[5], q[6], q[7]};
+    const R p = pres(q, rho, ir), b2 = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+    const R pt = p + R(0.5) * b2;
+    const R ub = u[0] * B[0] + u[1] * B[1] + u[2] * B[2];
+    const R un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2], bn = B[0] * n[0] + B[1] * n[1] + B[2] * n[2];
+    F[0] = rho * un;
+    for (int k = 0; k < 3; ++k) F[1 + k] = rho * u[k] * un + pt * n[k] - bn * B[k];
+    F[4] = un * (q[4] + pt) - bn * ub;
+    for (int k = 0; k < 3; ++k) F[5 + k] = un * B[k] - u[k] * bn + q[8] * n[k];
+    F[8] = ch * ch * bn;
+    const R cs2 = gamma * p * ir, ca2 = b2 * ir, can2 = bn * bn * ir;
+    R disc = (cs2 + ca2) * (cs2 + ca2) - R(4) * cs2 * can2;
+    disc = disc < R(0) ? R(0) : disc;
+    return rabs(un) + msqrt(R(0.5) * (cs2 + ca2 + msqrt(disc)));
+  }
+  template <int ND = 3>
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R FL[9], FR[9];
+    const R a = rmax(rmax(side(ql, n, FL), side(qr, n, FR)), ch);
+    rusanov_combine<R, 9>(FL, FR, ql, qr, a, f);
+  }
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    for (int k = 0; k < 9; ++k) g[k] = bc == BC_INFLOW ? inflow[k] : q[k];
+    if (bc == BC_WALL) {
+      reflect3(q, n, 1, 3, g);
+      reflect3(q, n, 5, 3, g);
+    }
+  }
+  MXM_HD bool source_active() const { return alpha != R(0); }
+  MXM_HD void source(const R *q, R *s) const {
+    for (int k = 0; k < 8; ++k) s[k] = R(0);
+    s[8] = -alpha * q[8];
+  }
+  template <class T>
+  MXM_HD void limit(T *) const {}
+};
+
+// src/two_fluid.mojo:52-513: electron (0..4) and ion (5..9) Euler fluids,
+// fields E (10..12), B (13..15), psi (16); Lorentz and current sources
+template <class R>
+struct TwoFluid3 {
+  static constexpr int NC = 17;
+  static constexpr bool kSource = true, kLimit = false;
+  R ge, gi, qme, qmi, ieps0, cl, ch, alpha, rmin, pmin, inflow[17];
+  MXM_HD static TwoFluid3 from(const double *p) {
+    TwoFluid3 t;
+    t.ge = R(p[0]), t.gi = R(p[1]), t.qme = R(p[2] / p[3]), t.qmi = R(p[4] / p[5]), t.ieps0 = R(1.0 / p[6]);
+    t.cl = R(p[7]), t.ch = R(p[8]), t.alpha = R(p[9]), t.rmin = R(p[10]), t.pmin = R(p[11]);
+    for (int k = 0; k < 17; ++k) t.inflow[k] = R(p[12 + k]);
+    return t;
+  }
+  MXM_HD R species_p(const R *q, R gamma, R ir) const {
+    const R p = (gamma - R(1)) * (q[4] - R(0.5) * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * ir);
+    return p > pmin ? p : pmin;
+  }
+  // normal flux of one species and |u_n| + c_s (src/two_fluid.mojo:112-131)
+  MXM_HD R species_side(const R *q, R gamma, const R *n, R *F) const {
+    const R rho = q[0] > rmin ? q[0] : rmin, ir = R(1) / rho;
+    const R p = species_p(q, gamma, ir);
+    const R u[3] = {q[1] * ir, q[2] * ir, q[3] * ir}, un = u[0] * n[0] + u[1] * n[1] + u[2] * n[2];
+    F[0] = rho * un;
+    for (int k = 0; k < 3; ++k) F[1 + k] = rho * u[k] * un + p * n[k];
+    F[4] = un * (q[4] + p);
+    return rabs(un) + msqrt(gamma * p * ir);
+  }
+  MXM_HD void fields(const R *q, const R *n, R *F) const {
+    const R c2 = cl * cl, ch2 = ch * ch;
+    const R *E = q + 10, *B = q + 13;
+    F[10] = c2 * (B[1] * n[2] - B[2] * n[1]);
+    F[11] = c2 * (B[2] * n[0] - B[0] * n[2]);
+    F[12] = c2 * (B[0] * n[1] - B[1] * n[0]);
+    F[13] = (n[1] * E[2] - n[2] * E[1]) + q[16] * n[0];
+    F[14] = (n[2] * E[0] - n[0] * E[2]) + q[16] * n[1];
+    F[15] = (n[0] * E[1] - n[1] * E[0]) + q[16] * n[2];
+    F[16] = ch2 * (B[0] * n[0] + B[1] * n[1] + B[2] * n[2]);
+  }
+  template <int ND>
+  MXM_HD void flux(const R *q, R *F) const {
+    for (int d = 0; d < ND; ++d) {
+      const R e[3] = {R(d == 0), R(d == 1), R(d == 2)};
+      R *Fd = F + 17 * d;
+      for (int s0 = 0; s0 < 10; s0 += 5) {
+        const R *qs = q + s0;
+        const R rho = qs[0] > rmin ? qs[0] : rmin, ir = R(1) / rho;
+        const R p = species_p(qs, s0 ? gi : ge, ir), m = qs[1 + d];
+        Fd[s0] = m;
+        for (int k = 0; k < 3; ++k) Fd[s0 + 1 + k] = m * (qs[1 + k] * ir);
+        Fd[s0 + 1 + d] += p;
+        Fd[s0 + 4] = (qs[1 + d] * ir) * (qs[4] + p);
+      }
+      fields(q, e, Fd);
+    }
+  }
+  template <int ND = 3>
+  MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    R FL[17], FR[17];
+    const R ae = rmax(species_side(ql, ge, n, FL), species_side(qr, ge, n, FR));
+    const R ai = rmax(species_side(ql + 5, gi, n, FL + 5), species_side(qr + 5, gi, n, FR + 5));
+    fields(ql, n, FL);
+    fields(qr, n, FR);
+    rusanov_combine<R, 17>(FL, FR, ql, qr, rmax(rmax(ae, ai), rmax(cl, ch)), f);
+  }
+  MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
+    for (int k = 0; k < 17; ++k) g[k] = bc == BC_INFLOW ? inflow[k] : q[k];
+    if (bc == BC_WALL) {
+      reflect3(q, n, 1, 3, g);
+      reflect3(q, n, 6, 3, g);
+      const R En = q[10] * n[0] + q[11] * n[1] + q[12] * n[2];
+      for (int d = 0; d < 3; ++d) g[10 + d] = R(2) * En * n[d] - q[10 + d];
+      reflect3(q, n, 13, 3, g);
+    }
+  }
+  MXM_HD bool source_active() const { return true; }
+  MXM_HD void source(const R *q, R *s) const {
+    const R re = q[0] > rmin ? q[0] : rmin, ri = q[5] > rmin ? q[5] : rmin;
+    const R ire = R(1) / re, iri = R(1) / ri;
+    const R ue[3] = {q[1] * ire, q[2] * ire, q[3] * ire}, ui[3] = {q[6] * iri, q[7] * iri, q[8] * iri};
+    const R *E = q + 10, *B = q + 13;
+    const R ve[3] = {ue[1] * B[2] - ue[2] * B[1], ue[2] * B[0] - ue[0] * B[2], ue[0] * B[1] - ue[1] * B[0]};
+    const R vi[3] = {ui[1] * B[2] - ui[2] * B[1], ui[2] * B[0] - ui[0] * B[2], ui[0] * B[1] - ui[1] * B[0]};
+    s[0] = s[5] = R(0);
+    for (int d = 0; d < 3; ++d) {
+      s[1 + d] = qme * re * (E[d] + ve[d]);
+      s[6 + d] = qmi * ri * (E[d] + vi[d]);
+      s[10 + d] = -(qme * re * ue[d] + qmi * ri * ui[d]) * ieps0;
+      s[13 + d] = R(0);
+    }
+    s[4] = qme * re * (ue[0] * E[0] + ue[1] * E[1] + ue[2] * E[2]);
+    s[9] = qmi * ri * (ui[0] * E[0] + ui[1] * E[1] + ui[2] * E[2]);
+    s[16] = -alpha * q[16];
+  }
+  template <class T>
+  MXM_HD void limit(T *) const {}
+};
+
 }  // namespace mxm

@@ -87,8 +87,8 @@ struct DevBuf {
 };
 
 inline int phys_nc(int phys) {
-  static const int nc[7] = {1, 5, 4, 3, 7, 6, 1};
-  return (phys >= 0 && phys < 7) ? nc[phys] : -1;
+  static const int nc[11] = {1, 5, 4, 3, 7, 6, 1, 3, 6, 9, 17};
+  return (phys >= 0 && phys < 11) ? nc[phys] : -1;
 }
 
 inline int phys_variant(int phys, const double *prm) {
@@ -106,7 +106,11 @@ inline int stage_cfg_default(int prec) { return prec == MXM_PREC_FP64 ? StageCfg
 inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec, int cfg) {
   const bool f64 = prec == MXM_PREC_FP64;
   if (cfg < 0 || cfg >= stage_cfg_count(prec)) return nullptr;
-  if (dim == 3) return phys <= MXM_PHYS_EULER ? (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg) : nullptr;
+  if (dim == 3) {
+    if (phys <= MXM_PHYS_EULER) return (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg);
+    return phys >= MXM_PHYS_SW ? (f64 ? stage_3d_sys_fp64 : stage_3d_sys_fp32)(phys, variant, P, cfg) : nullptr;
+  }
+  if (phys >= MXM_PHYS_SW) return nullptr;
   if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P, cfg);
   return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P, cfg);
 }
@@ -181,10 +185,10 @@ struct ElemTables {
   std::vector<int4> conn;  // limiter neighbours {partner slot, face, flags, 0}
 };
 
-inline ElemTables build_elem_tables(const HostMesh &m, const RefElem &re) {
+inline ElemTables build_elem_tables(const HostMesh &m, const RefElem &re, int nc) {
   ElemTables t;
   Connectivity cn = build_connectivity(m, re);
-  t.til = build_tiling(m, stage_epb(m.dim, m.NP), m.NFP);
+  t.til = build_tiling(m, stage_epb(m.dim, m.NP, m.NFP, nc), m.NFP);
   const int Kd = t.til.slots(), dd = m.dim * m.dim, NF = m.NF;
   t.invJ.assign((size_t)Kd * dd, 0.0);
   t.inv_vol.assign((size_t)Kd, 0.0);
@@ -215,7 +219,7 @@ inline OpTables build_operators(const RefElem &re, int dim, int NC, int prec) {
   OpTables o;
   const int NP = re.NP, NFP = re.NFP, NF = re.NF;
   if (prec == MXM_PREC_FP64) {
-    const int npad = stage_node_pad(dim, NP, NC);
+    const int npad = stage_node_pad(stage_epb(dim, NP, NFP, NC), NP, NC);
     o.Dt.assign((size_t)dim * NP * npad, 0.0);
     o.Lt.assign((size_t)NF * NFP * npad, 0.0);
     for (int d = 0; d < dim; ++d)

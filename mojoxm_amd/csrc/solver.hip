@@ -94,7 +94,7 @@ struct Solver {
 
   void build(const HostMesh &m, const RefElem &re) {
     dim = m.dim, P = m.P, NP = m.NP, NFP = m.NFP, NF = m.NF, K = m.K, Nf = m.Nf;
-    ElemTables t = build_elem_tables(m, re);
+    ElemTables t = build_elem_tables(m, re, NC);
     til = std::move(t.til);
     Kd = til.slots();
     ntiles = til.ntiles;
@@ -190,7 +190,7 @@ struct Solver {
     if (dout.n < (size_t)(3 * NC + 2)) dout.alloc(3 * NC + 2);
     auto fill = [&](auto &a) {
       a.invJ = invJ.p, a.inv_vol = inv_vol.p, a.w = w.p, a.Dn = Dn.p, a.tile_count = tile_count.p;
-      a.tile0 = 0, a.ntiles = ntiles, a.NC = NC, a.nvel = diag_nvel(phys);
+      a.epb = til.epb, a.tile0 = 0, a.ntiles = ntiles, a.NC = NC, a.nvel = diag_nvel(phys);
       a.part = dpart.p, a.pmax = dpart.p + (size_t)ntiles * kDiagSums * 2, a.out = dout.p;
     };
     if (prec == MXM_PREC_FP64) {
@@ -274,7 +274,8 @@ int create(int dim, const void *desc, int P, int phys, const double *prm_in, int
   if (P < 1 || P > 5) throw ArgError("P must be in 1..5");
   const int nc = phys_nc(phys);
   if (nc < 0) throw ArgError("unknown physics id");
-  if (dim == 3 && phys > MXM_PHYS_EULER) throw ArgError("2D module physics on a 3D mesh");
+  if (dim == 3 && phys > MXM_PHYS_EULER && phys < MXM_PHYS_SW) throw ArgError("2D module physics on a 3D mesh");
+  if (dim == 2 && phys >= MXM_PHYS_SW) throw ArgError("3D physics on a 2D mesh");
   auto s = std::make_unique<Solver>();
   s->phys = phys;
   s->NC = nc;
@@ -331,7 +332,7 @@ int mxm_get_constant(int which, int *value) {
       for (int c = 0; c < stage_cfg_count(prec); ++c)
         for (int P = 1; P <= 5; ++P)
           for (int dim : {2, 3})
-            for (int ph = 0; ph <= 6; ++ph)
+            for (int ph = 0; ph <= MXM_PHYS_TWOFLUID; ++ph)
               for (int v = 0; v < 8; ++v) {
                 // Euler variants: flux kind + 4 * entropy fix; Euler2D and SW: 2
                 const bool valid = ph == MXM_PHYS_EULER ? v != 4 : (ph == 2 || ph == 3) ? v < 2 : v == 0;
@@ -395,7 +396,7 @@ int mxm_mesh_create_3d(const mxm_mesh3d_desc *d, int P, void **mesh) {
     auto m = std::make_unique<HostMesh>(build_mesh_3d(d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->bc, P, re));
     // the kernel-side tables are validated here too, so host-only callers
     // exercise the same checks a solver runs
-    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->dim, m->NP), m->NFP));
+    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->dim, m->NP, m->NFP, 1), m->NFP));
     *mesh = m.release();
     return MXM_OK;
   });
@@ -409,7 +410,7 @@ int mxm_mesh_create_2d(const mxm_mesh2d_desc *d, int P, void **mesh) {
     check_bc(d->bc, 4);
     RefElem re = build_refelem(2, P);
     auto m = std::make_unique<HostMesh>(build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re));
-    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->dim, m->NP), m->NFP));
+    build_face_jobs(*m, re, build_connectivity(*m, re), build_tiling(*m, stage_epb(m->dim, m->NP, m->NFP, 1), m->NFP));
     *mesh = m.release();
     return MXM_OK;
   });
@@ -528,7 +529,7 @@ int mxm_dg_step_ssprk3(void *h, double dt, int nsteps) {
 int mxm_dg_enable_limiter(void *h, int on, double eps) {
   return guarded([&] {
     Solver *s = as_solver(h);
-    if (s->NC > kLimMaxNC) throw ArgError("limiter supports NC <= 8");
+    if (s->NC > kLimMaxNC) throw ArgError("limiter supports NC <= " + std::to_string(kLimMaxNC));
     s->limiter = on != 0;
     s->eps = eps;
     return MXM_OK;

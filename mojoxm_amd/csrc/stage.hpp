@@ -80,7 +80,7 @@ struct StageArgs {
   int ntiles;
 };
 
-constexpr int kPrmMax = 16;  // MXM_PRM_MAX
+constexpr int kPrmMax = 32;  // MXM_PRM_MAX
 
 // Launch configurations of the stage kernel: each is an occupancy target
 // (waves per SIMD, i.e. 256-thread blocks per CU) that bounds the register
@@ -103,12 +103,22 @@ struct StageCfg<float> {
 };
 constexpr int kStageCfgMax = 3;
 
-// Elements per 256-thread block for NP nodes per element, rounded down to
-// whole lattice cells (2 triangles, 6 tets) when a cell fits: tiles are
-// blocks of cells, so the LDS is sized for what a tile can hold.  The host
-// builds the tiles with the same value.
-constexpr int stage_epb(int ND, int NP) {
-  return 256 / NP >= (ND == 3 ? 6 : 2) ? 256 / NP - (256 / NP) % (ND == 3 ? 6 : 2) : 256 / NP;
+// Elements per 256-thread block: at most 256/NP for NP nodes per element,
+// rounded down to whole lattice cells (2 triangles, 6 tets) when a cell
+// fits, then reduced -- by cells, then to runs of elements -- until the
+// block's LDS at FP64 (the contraction operand, K rows of EPB*NC + 1
+// columns, and the geometry) fits 64 KB, which only the wide states (NC =
+// 9, 17) need.  Tiles are blocks of cells, so the LDS is sized for what a
+// tile can hold; the host builds the tiles with the same value.
+constexpr int stage_lds_bytes(int ND, int NP, int NFP, int NC, int epb) {
+  return (ND * NP + (ND + 1) * NFP) * (epb * NC + 1) * 8 + epb * (ND * ND + 1) * 8 + (ND + 1) * NFP * 4;
+}
+constexpr int stage_epb(int ND, int NP, int NFP, int NC) {
+  const int cell = ND == 3 ? 6 : 2;
+  int e = 256 / NP;
+  if (e >= cell) e -= e % cell;
+  while (e > 1 && stage_lds_bytes(ND, NP, NFP, NC, e) > 65536) e = e > cell ? e - cell : e - 1;
+  return e;
 }
 
 // FP64 contraction layout: the tile's (element, component) columns span
@@ -116,11 +126,11 @@ constexpr int stage_epb(int ND, int NP) {
 // producing `IH` consecutive output nodes, so the operator entries a wave
 // reads are wave-uniform.  The transposed operators are stored with
 // groups*IH (>= NP) zero-padded node columns.
-constexpr int stage_groups(int ND, int NP, int NC) {
-  return (stage_epb(ND, NP) * NC + 63) / 64 >= 4 ? 1 : 4 / ((stage_epb(ND, NP) * NC + 63) / 64);
+constexpr int stage_groups(int EPB, int NC) {
+  return (EPB * NC + 63) / 64 >= 4 ? 1 : 4 / ((EPB * NC + 63) / 64);
 }
-constexpr int stage_node_pad(int ND, int NP, int NC) {
-  return stage_groups(ND, NP, NC) * ((NP + stage_groups(ND, NP, NC) - 1) / stage_groups(ND, NP, NC));
+constexpr int stage_node_pad(int EPB, int NP, int NC) {
+  return stage_groups(EPB, NC) * ((NP + stage_groups(EPB, NC) - 1) / stage_groups(EPB, NC));
 }
 
 // FP32-MFMA operator shape: 16-row tiles of output nodes, K = ND*NP volume
@@ -128,13 +138,13 @@ constexpr int stage_node_pad(int ND, int NP, int NC) {
 constexpr int mfma_mtiles(int NP) { return (NP + 15) / 16; }
 constexpr int mfma_kpad(int ND, int NP, int NFP) { return (ND * NP + (ND + 1) * NFP + 3) / 4 * 4; }
 
-template <int ND, int P>
+template <int ND, int P, int NC = 1>
 struct Shape {
   static constexpr int NP = ND == 3 ? (P + 1) * (P + 2) * (P + 3) / 6 : (P + 1) * (P + 2) / 2;
   static constexpr int NFP = ND == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
   static constexpr int NF = ND + 1;
   static constexpr int BLOCK = 256;
-  static constexpr int EPB = stage_epb(ND, NP);
+  static constexpr int EPB = stage_epb(ND, NP, NFP, NC);
 };
 
 // Bijective map from the hardware block index (dispatched round-robin over
@@ -197,7 +207,7 @@ typedef float mxm_float4 __attribute__((ext_vector_type(4)));
 // (amdgpu_waves_per_eu); 1 leaves the register budget to the compiler.
 template <class Phys, int ND, int P, class R, int W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void dg_stage_kernel(StageArgs<R> A) {
-  using S = Shape<ND, P>;
+  using S = Shape<ND, P, Phys::NC>;
   constexpr bool kMFMA = std::is_same<R, float>::value;
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
   constexpr int TILE = EPB * NP * NC;  // values of one element tile
@@ -452,9 +462,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     // FP64 on the VALU: the (element, component) columns are the lanes;
     // each wave group produces IH consecutive output nodes, so the operator
     // entries are wave-uniform (scalar loads)
-    constexpr int GROUPS = stage_groups(ND, NP, NC);
+    constexpr int GROUPS = stage_groups(EPB, NC);
     constexpr int GSIZE = S::BLOCK / GROUPS;
-    constexpr int NPAD = stage_node_pad(ND, NP, NC);
+    constexpr int NPAD = stage_node_pad(EPB, NP, NC);
     constexpr int IH = NPAD / GROUPS;
     constexpr int ITER = (NCOL + GSIZE - 1) / GSIZE;  // columns per thread, at most
     const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
@@ -518,7 +528,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 
 template <class Phys, int ND, int P, class R, int W>
 hipError_t launch_stage(const void *args, hipStream_t s) {
-  using S = Shape<ND, P>;
+  using S = Shape<ND, P, Phys::NC>;
   const StageArgs<R> &a = *static_cast<const StageArgs<R> *>(args);
   const int nb = a.ntiles;
   if (nb <= 0) return hipSuccess;

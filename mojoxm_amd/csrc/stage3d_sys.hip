@@ -1,0 +1,8 @@
+// stage3d_sys.hip -- Family3DSystem stage kernels, plain FP64 form, P = 1..5.
+#include "stage_families.hpp"
+
+namespace mxm {
+StageLaunch stage_3d_sys_fp64(int phys, int variant, int P, int cfg) {
+  return pick_order<Family3DSystem, double>(phys, variant, P, cfg);
+}
+}  // namespace mxm

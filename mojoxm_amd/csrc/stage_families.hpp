@@ -64,6 +64,21 @@ struct Family2DModule {
   }
 };
 
+// the 3D systems with a Rusanov flux: shallow water, Maxwell, ideal MHD
+// with GLM cleaning, five-moment two-fluid
+struct Family3DSystem {
+  template <int P, class R, int W>
+  static StageLaunch pick(int phys, int) {
+    switch (phys) {
+      case 7: return &launch_stage<ShallowWater3<R>, 3, P, R, W>;
+      case 8: return &launch_stage<Maxwell3<R>, 3, P, R, W>;
+      case 9: return &launch_stage<IdealMHD3<R>, 3, P, R, W>;
+      case 10: return &launch_stage<TwoFluid3<R>, 3, P, R, W>;
+    }
+    return nullptr;
+  }
+};
+
 template <class F, int P, class R>
 StageLaunch pick_cfg(int phys, int variant, int cfg) {
   switch (cfg) {
@@ -90,6 +105,8 @@ StageLaunch pick_order(int phys, int variant, int P, int cfg) {
 // one entry per (family, form), defined in the stage*.hip units
 StageLaunch stage_3d_fp64(int phys, int variant, int P, int cfg);
 StageLaunch stage_3d_fp32(int phys, int variant, int P, int cfg);
+StageLaunch stage_3d_sys_fp64(int phys, int variant, int P, int cfg);
+StageLaunch stage_3d_sys_fp32(int phys, int variant, int P, int cfg);
 StageLaunch stage_2d_trait_fp64(int phys, int variant, int P, int cfg);
 StageLaunch stage_2d_trait_fp32(int phys, int variant, int P, int cfg);
 StageLaunch stage_2d_module_fp64(int phys, int variant, int P, int cfg);

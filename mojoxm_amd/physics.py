@@ -10,6 +10,11 @@ Trait physics (usable on tets and triangles):
   MHDGLM2D       src/local_mesh_2d_gpu_mhd_glm.mojo    (Rusanov, psi damping)
   Maxwell2D      src/local_mesh_2d_gpu_maxwell.mojo    (Rusanov, PEC walls)
   Advection2D    src/local_mesh_2d_gpu_advection.mojo  (upwind)
+Other trait physics of the 3D solver (tets only, Rusanov fluxes):
+  ShallowWater        src/shallow_water.mojo (2D system embedded in 3D)
+  Maxwell             src/maxwell.mojo (PEC walls, J and M sources)
+  IdealMHD            src/mhd.mojo (GLM cleaning, damping as a source)
+  FiveMomentTwoFluid  src/two_fluid.mojo (electron + ion fluids, fields)
 """
 from __future__ import annotations
 
@@ -17,12 +22,13 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-PHYS_ADVECTION, PHYS_EULER, PHYS_EULER2D, PHYS_SW2D, PHYS_MHDGLM2D, PHYS_MAXWELL2D, PHYS_ADVECTION2D = range(7)
+(PHYS_ADVECTION, PHYS_EULER, PHYS_EULER2D, PHYS_SW2D, PHYS_MHDGLM2D, PHYS_MAXWELL2D, PHYS_ADVECTION2D, PHYS_SW, PHYS_MAXWELL,
+ PHYS_MHD, PHYS_TWOFLUID) = range(11)
 FLUX_RUSANOV, FLUX_ROE, FLUX_HLLE, FLUX_HLLEC = range(4)
 
 
 def _pack(vals) -> np.ndarray:
-    out = np.zeros(16)
+    out = np.zeros(32)
     v = np.asarray(vals, dtype=np.float64)
     out[: v.size] = v
     return out
@@ -135,3 +141,76 @@ class Advection2D:
 
     def params(self) -> np.ndarray:
         return _pack([self.vx, self.vy, self.inflow_q])
+
+
+@dataclass
+class ShallowWater:
+    """src/shallow_water.mojo:31-52: state (h, hu, hv), F^z = 0."""
+    g: float = 1.0
+    h_min: float = 1e-6
+    inflow: tuple = (0.0, 0.0, 0.0)
+    phys_id = PHYS_SW
+    NUM_COMPONENTS = 3
+    dims = (3,)
+
+    def params(self) -> np.ndarray:
+        return _pack([self.g, self.h_min, *self.inflow])
+
+
+@dataclass
+class Maxwell:
+    """src/maxwell.mojo:48-110: state (E, B), uniform sources J and M."""
+    c: float = 1.0
+    J: tuple = (0.0, 0.0, 0.0)
+    M: tuple = (0.0, 0.0, 0.0)
+    inflow: tuple = (0.0,) * 6
+    phys_id = PHYS_MAXWELL
+    NUM_COMPONENTS = 6
+    dims = (3,)
+
+    def params(self) -> np.ndarray:
+        return _pack([self.c, *self.J, *self.M, *self.inflow])
+
+
+@dataclass
+class IdealMHD:
+    """src/mhd.mojo:197-252: state (rho, rho u[3], E, B[3], psi); GLM with
+    speed c_h and damping alpha_d (a source term)."""
+    gamma: float = 5.0 / 3.0
+    min_density: float = 1e-6
+    min_pressure: float = 1e-6
+    c_h: float = 0.0
+    alpha_d: float = 0.0
+    inflow: tuple = (0.0,) * 9
+    phys_id = PHYS_MHD
+    NUM_COMPONENTS = 9
+    dims = (3,)
+
+    def params(self) -> np.ndarray:
+        return _pack([self.gamma, self.min_density, self.min_pressure, self.c_h, self.alpha_d, *self.inflow])
+
+
+@dataclass
+class FiveMomentTwoFluid:
+    """src/two_fluid.mojo:140-210: state (electron rho, m[3], E; ion rho,
+    m[3], E; field E[3], B[3], psi)."""
+    gamma_e: float = 5.0 / 3.0
+    gamma_i: float = 5.0 / 3.0
+    q_e: float = -1.0
+    m_e: float = 1.0 / 25.0
+    q_i: float = 1.0
+    m_i: float = 1.0
+    eps0: float = 1.0
+    c_light: float = 1.0
+    c_h: float = 0.0
+    alpha_d: float = 0.0
+    min_density: float = 1e-6
+    min_pressure: float = 1e-6
+    inflow: tuple = (0.0,) * 17
+    phys_id = PHYS_TWOFLUID
+    NUM_COMPONENTS = 17
+    dims = (3,)
+
+    def params(self) -> np.ndarray:
+        return _pack([self.gamma_e, self.gamma_i, self.q_e, self.m_e, self.q_i, self.m_i, self.eps0, self.c_light,
+                      self.c_h, self.alpha_d, self.min_density, self.min_pressure, *self.inflow])

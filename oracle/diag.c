@@ -17,9 +17,9 @@
  * with u = m / rho at the nodes and the velocity gradient from the nodal
  * differentiation matrices (orc_nodal_deriv) mapped by invJ (rows =
  * reference directions, as in the stage).  Physics with a velocity:
- * EULER (rho, m[3], E), EULER2D (rho, m[2], E), MHDGLM2D (rho, m[2], ...)
- * and SW2D (h, h u[2]: h in place of rho); the others report NaN for the
- * two.  Sums are carried in long double here (the device form uses
+ * EULER and MHD (rho, m[3], ...), EULER2D (rho, m[2], E), MHDGLM2D (rho,
+ * m[2], ...), SW2D and SW (h, h u[2]: h in place of rho); the others
+ * report NaN for the two.  Sums are carried in long double here (the device form uses
  * compensated FP64 sums), so this is the reference value to ~1e-18.
  *
  * Output layout, NC = the physics' component count:
@@ -45,10 +45,12 @@ typedef long double real;
 /* Velocity component index base and density index of a physics, or -1. */
 static int velocity_layout(int phys, int *nv) {
     switch (phys) {
-    case ORC_PHYS_EULER: *nv = 3; return 1;
+    case ORC_PHYS_EULER:
+    case ORC_PHYS_MHD: *nv = 3; return 1;
     case ORC_PHYS_EULER2D:
     case ORC_PHYS_MHDGLM2D:
-    case ORC_PHYS_SW2D: *nv = 2; return 1;
+    case ORC_PHYS_SW2D:
+    case ORC_PHYS_SW: *nv = 2; return 1;
     default: *nv = 0; return -1;
     }
 }

@@ -70,7 +70,11 @@ enum {
     ORC_PHYS_SW2D = 3,      /* local_mesh_2d_gpu_sw        NC=3 */
     ORC_PHYS_MHDGLM2D = 4,  /* local_mesh_2d_gpu_mhd_glm   NC=7 */
     ORC_PHYS_MAXWELL2D = 5, /* local_mesh_2d_gpu_maxwell   NC=6 */
-    ORC_PHYS_ADVECTION2D = 6 /* local_mesh_2d_gpu_advection NC=1 */
+    ORC_PHYS_ADVECTION2D = 6, /* local_mesh_2d_gpu_advection NC=1 */
+    ORC_PHYS_SW = 7,         /* src/shallow_water.mojo      NC=3 */
+    ORC_PHYS_MAXWELL = 8,    /* src/maxwell.mojo            NC=6 */
+    ORC_PHYS_MHD = 9,        /* src/mhd.mojo (GLM)          NC=9 */
+    ORC_PHYS_TWOFLUID = 10   /* src/two_fluid.mojo          NC=17 */
 };
 int orc_phys_nc(int phys);
 /* Pointwise hooks of the Physics trait (src/solver.mojo:97-145) for the
@@ -84,6 +88,16 @@ int orc_boundary_flux(int phys, const double *prm, const double *q, int bc,
                       double nx, double ny, double nz, double *flux,
                       double *speed);
 int orc_limit_state(int phys, const double *prm, double *q);
+/* The other trait physics (physics3d.c): same hook shapes, flux[d*NC+c]
+ * for the internal flux; orc3_source = source_term. */
+int orc3_phys_nc(int phys);
+int orc3_internal_flux(int phys, const double *prm, const double *q, double *flux);
+int orc3_numerical_flux(int phys, const double *prm, const double *ql, const double *qr, double nx, double ny,
+                        double nz, double *flux, double *speed);
+int orc3_boundary_flux(int phys, const double *prm, const double *q, int bc, double nx, double ny, double nz,
+                       double *flux, double *speed);
+int orc3_ghost(int phys, const double *prm, const double *q, int bc, double nx, double ny, double nz, double *g);
+int orc3_source(int phys, const double *prm, const double *q, double *s);
 /* Face flux of a 2D module physics at one face node (ghost state built
  * from bc exactly as the module's face kernel does). */
 int orc_face_flux_2d(int phys, const double *prm, const double *ql,

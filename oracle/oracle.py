@@ -16,7 +16,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-PHYS_ADVECTION, PHYS_EULER, PHYS_EULER2D, PHYS_SW2D, PHYS_MHDGLM2D, PHYS_MAXWELL2D, PHYS_ADVECTION2D = range(7)
+(PHYS_ADVECTION, PHYS_EULER, PHYS_EULER2D, PHYS_SW2D, PHYS_MHDGLM2D, PHYS_MAXWELL2D, PHYS_ADVECTION2D, PHYS_SW, PHYS_MAXWELL,
+ PHYS_MHD, PHYS_TWOFLUID) = range(11)
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -76,6 +77,7 @@ def lib():
         L.orc_step_ssprk3.argtypes = [C.POINTER(_Mesh), C.POINTER(_Ops), C.c_int, _dp, _dp, C.c_double,
                                       C.c_int, C.c_int, C.c_double]
         L.orc_nodal_deriv.argtypes = [C.c_int, C.c_int, _dp]
+        L.orc3_source.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_diagnostics.argtypes = [C.POINTER(_Mesh), _dp, _dp, C.c_int, _dp, _dp]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
@@ -234,7 +236,7 @@ def _f(a):
 
 
 def _prm(p):
-    out = np.zeros(16)
+    out = np.zeros(32)
     p = np.asarray(p, dtype=np.float64).ravel()
     out[: p.size] = p
     return out
@@ -268,6 +270,15 @@ def internal_flux(phys, prm, q):
     if lib().orc_internal_flux(phys, _prm(prm), _f(q), f):
         raise RuntimeError("internal_flux")
     return f.reshape(3, nc)
+
+
+def source(phys, prm, q):
+    """source_term of the other trait physics (physics3d.c)."""
+    nc = lib().orc_phys_nc(phys)
+    s = np.zeros(nc)
+    if lib().orc3_source(phys, _prm(prm), _f(q), s):
+        raise RuntimeError("source")
+    return s
 
 
 def limit_state(phys, prm, q):

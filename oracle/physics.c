@@ -36,7 +36,7 @@ int orc_phys_nc(int phys) {
     case ORC_PHYS_MAXWELL2D: return 6;
     case ORC_PHYS_ADVECTION2D: return 1;
     }
-    return -1;
+    return orc3_phys_nc(phys);
 }
 
 /* ---------------------------------------------------------- Euler ---- */
@@ -371,7 +371,7 @@ int orc_internal_flux(int phys, const double *prm, const double *q, double *flux
         flux[14] = w * (q[4] + p);
         return 0;
     }
-    return -1;
+    return orc3_internal_flux(phys, prm, q, flux);
 }
 
 int orc_numerical_flux(int phys, const double *prm, const double *ql, const double *qr, double nx,
@@ -386,7 +386,7 @@ int orc_numerical_flux(int phys, const double *prm, const double *ql, const doub
         euler_t e = euler_params(prm);
         sp = euler_nflux(&e, ql, qr, nx, ny, nz, flux);
     } else {
-        return -1;
+        return orc3_numerical_flux(phys, prm, ql, qr, nx, ny, nz, flux, speed);
     }
     if (speed) *speed = sp;
     return 0;
@@ -423,7 +423,7 @@ int orc_boundary_flux(int phys, const double *prm, const double *q, int bc, doub
         if (speed) *speed = sp;
         return 0;
     }
-    return -1;
+    return orc3_boundary_flux(phys, prm, q, bc, nx, ny, nz, flux, speed);
 }
 
 /* Euler.limit_state, src/euler.mojo:758-792 */

@@ -55,12 +55,13 @@ int orc_get_threads(void) {
 #endif
 }
 
-#define MAXNC 8
+#define MAXNC 17
 #define MAXNP 64
 
-static int is_trait(int phys) { return phys == ORC_PHYS_ADVECTION || phys == ORC_PHYS_EULER; }
+static int is_trait(int phys) { return phys == ORC_PHYS_ADVECTION || phys == ORC_PHYS_EULER || orc3_phys_nc(phys) > 0; }
 
-/* Trait source_term: Advection none; Euler gravity, src/euler.mojo:738-747. */
+/* Trait source_term: Advection none; Euler gravity, src/euler.mojo:738-747;
+ * the other trait physics, physics3d.c. */
 static void trait_source(int phys, const double *prm, const double *q, double *s) {
     if (phys == ORC_PHYS_EULER) {
         s[0] = 0.0;
@@ -68,6 +69,8 @@ static void trait_source(int phys, const double *prm, const double *q, double *s
         s[2] = q[0] * prm[6];
         s[3] = q[0] * prm[7];
         s[4] = q[1] * prm[5] + q[2] * prm[6] + q[3] * prm[7];
+    } else if (orc3_phys_nc(phys) > 0) {
+        orc3_source(phys, prm, q, s);
     } else {
         s[0] = 0.0;
     }

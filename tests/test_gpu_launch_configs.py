@@ -26,6 +26,9 @@ CASES = [
     ((5, 4), (1.0, 1.0), ph.Maxwell2D(), 2),
     ((2, 3, 2), (1.0, 1.0, 1.0), ph.Euler(flux_type=ph.FLUX_RUSANOV), 3),
     ((2, 2, 2), (1.0, 1.0, 1.0), ph.Advection(1.0, -0.5, 0.3), 2),
+    ((2, 2, 2), (1.0, 1.0, 1.0), ph.Maxwell(c=1.2, J=(0.1, 0.0, -0.1)), 3),
+    ((2, 2, 2), (1.0, 1.0, 1.0), ph.IdealMHD(c_h=1.2, alpha_d=2.0), 2),
+    ((2, 2, 2), (1.0, 1.0, 1.0), ph.FiveMomentTwoFluid(c_light=2.0, c_h=1.5, alpha_d=0.5), 2),
 ]
 
 

@@ -51,6 +51,31 @@ def random_state(phys, K, NP, rng, ndim):
         return np.stack([rho, rho * u[..., 0], rho * u[..., 1], B[..., 0], B[..., 1], E, psi], -1)
     if isinstance(phys, ph.Maxwell2D):
         return rng.uniform(-1.0, 1.0, (K, NP, 6))
+    if isinstance(phys, ph.ShallowWater):
+        h = rng.uniform(0.8, 1.2, (K, NP))
+        u = rng.uniform(-0.3, 0.3, (K, NP, 2))
+        return np.stack([h, h * u[..., 0], h * u[..., 1]], -1)
+    if isinstance(phys, ph.Maxwell):
+        return rng.uniform(-1.0, 1.0, (K, NP, 6))
+    if isinstance(phys, ph.IdealMHD):
+        rho = rng.uniform(0.8, 1.2, (K, NP))
+        u = rng.uniform(-0.2, 0.2, (K, NP, 3))
+        B = rng.uniform(-0.5, 0.5, (K, NP, 3))
+        p = rng.uniform(0.8, 1.2, (K, NP))
+        E = p / (phys.gamma - 1) + 0.5 * rho * (u ** 2).sum(-1) + 0.5 * (B ** 2).sum(-1)
+        psi = rng.uniform(-0.1, 0.1, (K, NP))
+        return np.concatenate([rho[..., None], rho[..., None] * u, E[..., None], B, psi[..., None]], -1)
+    if isinstance(phys, ph.FiveMomentTwoFluid):
+        q = np.zeros((K, NP, 17))
+        for s0, gam in ((0, phys.gamma_e), (5, phys.gamma_i)):
+            rho = rng.uniform(0.8, 1.2, (K, NP))
+            u = rng.uniform(-0.2, 0.2, (K, NP, 3))
+            p = rng.uniform(0.8, 1.2, (K, NP))
+            q[..., s0] = rho
+            q[..., s0 + 1:s0 + 4] = rho[..., None] * u
+            q[..., s0 + 4] = p / (gam - 1) + 0.5 * rho * (u ** 2).sum(-1)
+        q[..., 10:17] = rng.uniform(-0.5, 0.5, (K, NP, 7))
+        return q
     raise TypeError(phys)
 
 
