@@ -68,6 +68,20 @@ struct BGroup {
   std::vector<int> packed;   // the member list the device arena holds
   bool dirty = true;
   int ntiles = 0, maxj = 0;
+  // The arena orders the members by their remaining steps at the repack,
+  // longest first, so the instances still running always hold a prefix of
+  // the tiles.  rem[k] / end_tile[k]: member k's remaining steps and the
+  // end of its tiles; launched: steps enqueued since the repack; the graphs
+  // launch grid_tiles blocks, shrunk as the running prefix shrinks.
+  std::vector<int> rem, end_tile;
+  long long launched = 0;
+  int grid_tiles = 0;
+  // tiles of the members still running after s more steps
+  int active_tiles(long long s) const {
+    int t = 0;
+    for (size_t k = 0; k < rem.size() && rem[k] > s; ++k) t = end_tile[k];
+    return t;
+  }
   DevBuf<double> qd[3];
   DevBuf<float> qf[3];
   DevBuf<double> mean, invJ, inv_vol, jgeo, Dt, Lt, w, iprm, idt, ipost;
@@ -260,6 +274,10 @@ struct Batch {
     MXM_HIP(hipStreamSynchronize(stream));
     pull_counters(g);
     g.drop_graph();
+    std::stable_sort(g.members.begin(), g.members.end(), [&](int a, int b) {
+      const BInst &x = inst(a), &y = inst(b);
+      return x.nsteps - x.steps_done > y.nsteps - y.steps_done;
+    });
     std::vector<int> old_base(g.members.size());
     int tiles = 0, maxj = 1;
     for (size_t k = 0; k < g.members.size(); ++k) {
@@ -338,7 +356,24 @@ struct Batch {
     if (g.limiter) g.mean.alloc(slots * g.NC);
     g.packed = g.members;
     g.dirty = false;
-    if (g.ntiles > 0) {
+    g.rem.resize(ni);
+    g.end_tile.resize(ni);
+    for (size_t k = 0; k < ni; ++k) {
+      const BInst &in = inst(g.members[k]);
+      g.rem[k] = std::max(0, in.nsteps - in.steps_done);
+      g.end_tile[k] = in.base_tile + in.t.til.ntiles;
+    }
+    g.launched = 0;
+    recapture(g, g.active_tiles(0));
+  }
+
+  // graphs over the first `tiles` tiles of the arena (none when 0); the
+  // old graphs finish before they are destroyed
+  void recapture(BGroup &g, int tiles) {
+    MXM_HIP(hipStreamSynchronize(stream));
+    g.drop_graph();
+    g.grid_tiles = tiles;
+    if (tiles > 0) {
       if (tune) choose_config(g);
       capture(g);
     }
@@ -349,7 +384,7 @@ struct Batch {
   // which the next step rewrites before reading)
   void choose_config(BGroup &g) {
     int oct = 0;
-    while ((1 << oct) < g.ntiles) ++oct;
+    while ((1 << oct) < g.grid_tiles) ++oct;
     const auto key = std::make_pair(g.key, oct);
     auto it = tuned.find(key);
     if (it == tuned.end()) {
@@ -382,7 +417,7 @@ struct Batch {
     a.tile_count = g.tile_count.p;
     a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p;
     a.Dt = g.Dt.p, a.Lt = g.Lt.p, a.Aop = g.Aop.p, a.fmap = g.fmap.p;
-    a.jobs = g.jobs.p, a.jgeo = g.jgeo.p, a.ipr = g.ipr.p, a.maxj = g.maxj, a.ntiles = g.ntiles;
+    a.jobs = g.jobs.p, a.jgeo = g.jgeo.p, a.ipr = g.ipr.p, a.maxj = g.maxj, a.ntiles = g.grid_tiles;
     return a;
   }
 
@@ -393,7 +428,7 @@ struct Batch {
       const StageArgs<R> a = stage_args(g, q, s);
       MXM_HIP(g.launch(&a, stream));
       if (g.limiter)
-        MXM_HIP(launch_limiter<R>(a.q_out, g.mean.p, g.w.p, g.conn.p, g.ntiles * g.epb, g.NP, g.NC, g.NF, g.dim,
+        MXM_HIP(launch_limiter<R>(a.q_out, g.mean.p, g.w.p, g.conn.p, g.grid_tiles * g.epb, g.NP, g.NC, g.NF, g.dim,
                                   g.eps, stream, g.tile_inst.p, g.iactive.p, g.epb));
     }
     hipLaunchKernelGGL(bump_steps_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, g.isteps.p, g.insteps.p,
@@ -436,15 +471,21 @@ struct Batch {
   void step(int n) {
     for (auto &kv : groups)
       if (kv.second->dirty) repack(*kv.second);
-    auto launch_all = [&](int which) {
+    // instances that finish inside a launch are switched off by their
+    // device counters; the grid shrinks to the running prefix between
+    // launches once it has lost an eighth of its tiles
+    auto launch_all = [&](int which, int k) {
       for (auto &kv : groups) {
         BGroup &g = *kv.second;
-        if (g.ntiles == 0) continue;
+        const int need = g.active_tiles(g.launched);
+        g.launched += k;
+        if (need == 0) continue;
+        if (need <= g.grid_tiles - g.grid_tiles / 8) recapture(g, need);
         MXM_HIP(hipGraphLaunch(g.exec[which], stream));
       }
     };
-    for (; n >= BGroup::kGraphSteps; n -= BGroup::kGraphSteps) launch_all(0);
-    for (; n > 0; --n) launch_all(1);
+    for (; n >= BGroup::kGraphSteps; n -= BGroup::kGraphSteps) launch_all(0, BGroup::kGraphSteps);
+    for (; n > 0; --n) launch_all(1, 1);
   }
 
   void sync_counters() {

@@ -89,6 +89,44 @@ def test_batch_joins_and_leaves():
         assert np.array_equal(b.get_state(d), alone(n, L, phys, P, q[2], dt, 3, M.PREC_FP64))
 
 
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_FP32_MFMA])
+def test_batch_running_prefix_shrinks(precision):
+    """Many instances of one group with step counts from 1 to 30: the arena
+    is ordered longest-running first and the launch grid shrinks to the
+    running prefix between launches (several times here, in chunks of 1, 3
+    and 8 steps, with one instance joining mid-run).  Every instance must
+    still equal its solve alone, and finished ones keep their state."""
+    rng = np.random.default_rng(21)
+    phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
+    sizes = [(3, 3), (5, 4), (4, 6), (7, 3), (2, 2), (6, 6), (3, 5), (8, 2), (4, 4), (5, 5), (2, 7), (6, 3)]
+    steps = [30, 1, 7, 13, 2, 24, 5, 17, 9, 3, 28, 11]
+    dt, P = 1e-3, 2
+    cases = []
+    with Batch(precision) as b:
+        ids = []
+        for nn, ns in zip(sizes, steps):
+            q0 = random_state(phys, 2 * nn[0] * nn[1], 6, rng, 2)
+            if precision == M.PREC_FP32_MFMA:
+                q0 = q0.astype(np.float32).astype(np.float64)
+            cases.append((nn, q0, ns))
+            ids.append(b.add(nn, (1.0, 1.0), phys, P, q0, dt, ns))
+        done = set()
+        for chunk in (1, 3, 8, 1, 8, 3):
+            b.step(chunk)
+            done |= set(b.poll())
+        q0 = random_state(phys, 2 * 5 * 3, 6, rng, 2)
+        if precision == M.PREC_FP32_MFMA:
+            q0 = q0.astype(np.float32).astype(np.float64)
+        cases.append(((5, 3), q0, 6))
+        ids.append(b.add((5, 3), (1.0, 1.0), phys, P, q0, dt, 6))
+        for chunk in (8, 1, 3):
+            b.step(chunk)
+            done |= set(b.poll())
+        assert done == set(ids)
+        for i, (nn, q, ns) in zip(ids, cases):
+            assert np.array_equal(b.get_state(i), alone(nn, (1.0, 1.0), phys, P, q, dt, ns, precision)), (nn, ns)
+
+
 def test_batch_against_oracle(oracle):
     rng = np.random.default_rng(9)
     phys = ph.MHDGLM2D(c_h=1.3, alpha_d=1.3 / 0.18)
