@@ -230,14 +230,15 @@ inline OpTables build_operators(const RefElem &re, int dim, int NC, int prec) {
         for (int r = 0; r < NFP; ++r)
           o.Lt[((size_t)fc * NFP + r) * npad + i] = re.Lift[((size_t)fc * NP + i) * NFP + r];
   } else {
-    const int mt = mfma_mtiles(NP), kp = mfma_kpad(dim, NP, NFP), kv = dim * NP;
+    // columns: D_0 .. D_dim-1 (dim*NP, zero-padded to kvp), then Lift
+    const int mt = mfma_mtiles(NP), kp = mfma_kpad(dim, NP, NFP), kvp = mfma_kvpad(dim, NP);
     o.Aop.assign((size_t)mt * 16 * kp, 0.f);
     for (int i = 0; i < NP; ++i) {
       for (int d = 0; d < dim; ++d)
         for (int j = 0; j < NP; ++j) o.Aop[(size_t)i * kp + d * NP + j] = float(re.D[((size_t)d * NP + i) * NP + j]);
       for (int fc = 0; fc < NF; ++fc)
         for (int r = 0; r < NFP; ++r)
-          o.Aop[(size_t)i * kp + kv + fc * NFP + r] = float(re.Lift[((size_t)fc * NP + i) * NFP + r]);
+          o.Aop[(size_t)i * kp + kvp + fc * NFP + r] = float(re.Lift[((size_t)fc * NP + i) * NFP + r]);
     }
   }
   return o;

@@ -106,12 +106,16 @@ constexpr int kStageCfgMax = 3;
 // Elements per 256-thread block: at most 256/NP for NP nodes per element,
 // rounded down to whole lattice cells (2 triangles, 6 tets) when a cell
 // fits, then reduced -- by cells, then to runs of elements -- until the
-// block's LDS at FP64 (the contraction operand, K rows of EPB*NC + 1
-// columns, and the geometry) fits 64 KB, which only the wide states (NC =
-// 9, 17) need.  Tiles are blocks of cells, so the LDS is sized for what a
+// block's LDS at FP64 (the operand array -- the q_in tile and the face
+// fluxes, later the ND*NP flux planes, rows of EPB*NC + 1 values -- and
+// the geometry) fits 64 KB, which only the two-fluid state (NC = 17)
+// needs.  Tiles are blocks of cells, so the LDS is sized for what a
 // tile can hold; the host builds the tiles with the same value.
 constexpr int stage_lds_bytes(int ND, int NP, int NFP, int NC, int epb) {
-  return (ND * NP + (ND + 1) * NFP) * (epb * NC + 1) * 8 + epb * (ND * ND + 1) * 8 + (ND + 1) * NFP * 4;
+  return (epb * NP * NC + (ND + 1) * NFP * (epb * NC + 1) > ND * NP * (epb * NC + 1)
+              ? epb * NP * NC + (ND + 1) * NFP * (epb * NC + 1)
+              : ND * NP * (epb * NC + 1)) * 8 +
+         epb * (ND * ND + 1) * 8 + (ND + 1) * NFP * 4;
 }
 constexpr int stage_epb(int ND, int NP, int NFP, int NC) {
   const int cell = ND == 3 ? 6 : 2;
@@ -136,7 +140,20 @@ constexpr int stage_node_pad(int EPB, int NP, int NC) {
 // FP32-MFMA operator shape: 16-row tiles of output nodes, K = ND*NP volume
 // columns then NF*NFP lift columns, padded to the MFMA depth of 4.
 constexpr int mfma_mtiles(int NP) { return (NP + 15) / 16; }
-constexpr int mfma_kpad(int ND, int NP, int NFP) { return (ND * NP + (ND + 1) * NFP + 3) / 4 * 4; }
+// FP32-MFMA operator columns: the ND*NP derivative columns padded to a
+// multiple of 4, then the (ND+1)*NFP lift columns padded to a multiple of 4
+constexpr int mfma_kvpad(int ND, int NP) { return (ND * NP + 3) / 4 * 4; }
+constexpr int mfma_kpad(int ND, int NP, int NFP) { return mfma_kvpad(ND, NP) + ((ND + 1) * NFP + 3) / 4 * 4; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+// Length of the stage kernel's LDS operand array (in values): the q_in
+// tile [0, TILE) with the face fluxes after it (NFF rows of NCOLS), and
+// later the KV flux-plane rows from 0; the MFMA form also reads its
+// padding rows and up to NTC columns of every row.
+constexpr int stage_operand_len(bool mfma, int TILE, int NCOLS, int KV, int KVP, int NFF, int KPAD, int NTC) {
+  return mfma ? cmax(cmax(TILE + NFF * NCOLS, KVP * NCOLS),
+                     cmax(TILE + (KPAD - KVP - 1) * NCOLS + NTC, (KVP - 1) * NCOLS + NTC))
+              : cmax(TILE + NFF * NCOLS, KV * NCOLS);
+}
 
 template <int ND, int P, int NC = 1>
 struct Shape {
@@ -216,31 +233,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   // padding, so that the node threads' row-strided writes of the flux
   // planes spread over the LDS banks.
   constexpr int NCOLS = NCOL + 1;
-  // The contraction's right operand: KV = ND*NP flux-plane rows, then
-  // NF*NFP face-flux rows, one array.  FP32-MFMA: the product runs over
-  // KPAD rows in 4-deep steps and NT 16-column tiles; the array is long
-  // enough for every such read, and rows K..KPAD-1 hold zeros (their
-  // operator columns are zero; a column beyond NCOL only feeds an output
-  // column that is never stored).
-  constexpr int KV = ND * NP, K = KV + NF * NFP;
-  constexpr int KPAD = kMFMA ? mfma_kpad(ND, NP, NFP) : K;
+  // The contraction's right operands: NFF = NF*NFP face-flux rows, then
+  // KV = ND*NP flux-plane rows in the same array (stage_operand_len).
+  // FP32-MFMA: the products run over 4-deep steps of operator columns and
+  // NT 16-column tiles; the array is long enough for every such read, and
+  // the padding rows (face fluxes NFF..KPAD-KVP-1, planes KV..KVP-1) hold
+  // zeros (their operator columns are zero; a column beyond NCOL only
+  // feeds an output column that is never stored).
+  constexpr int NFF = NF * NFP, KV = ND * NP;
+  constexpr int KVP = kMFMA ? mfma_kvpad(ND, NP) : KV;
+  constexpr int KPAD = kMFMA ? mfma_kpad(ND, NP, NFP) : KV + NFF;
   constexpr int MT = mfma_mtiles(NP);
   constexpr int NT = (NCOL + 15) / 16;
-  constexpr int NB = kMFMA && (KPAD - 1) * NCOLS + NT * 16 > K * NCOLS ? (KPAD - 1) * NCOLS + NT * 16 : K * NCOLS;
+  constexpr int NB = stage_operand_len(kMFMA, TILE, NCOLS, KV, KVP, NFF, KPAD, NT * 16);
   // FP32-MFMA: the operator [D | Lift] staged in LDS when it is at most
   // 16 KB (2D, 3D P <= 3), else read from global memory (L2) in the loop
   constexpr int NA = kMFMA ? MT * 16 * KPAD : 0;
   constexpr bool kStageA = kMFMA && NA * 4 <= 16384;
-  // sW = rows 0..KV-1: first the q_in tile as [el][j][c] (TILE contiguous
-  // values); after the face phase the contravariant nodal fluxes as
-  // [k][j][el][c] (row kj at kj*NCOLS); after the contraction (volume -
-  // lift) as [el][i][c].  sS = rows KV..K-1: face fluxes [f*NFP + r][el][c].
+  // sW = sB: first the q_in tile as [el][j][c] (TILE contiguous values);
+  // after the lift product the contravariant nodal fluxes as [k][j][el][c]
+  // (row kj at kj*NCOLS); after the volume product the result as
+  // [el][i][c].  sS = sB + TILE: face fluxes [f*NFP + r][el][c].
   __shared__ R sB[NB];
   __shared__ float sA[kStageA ? NA : 1];
   __shared__ double sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
   R *const sW = sB;
-  R *const sS = sB + KV * NCOLS;
+  R *const sS = sB + TILE;
 
   const int tid = threadIdx.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -313,7 +332,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
       if (k * S::BLOCK + tid < NA) sA[k * S::BLOCK + tid] = ta[k];
   }
   if constexpr (kMFMA)
-    for (int t = K * NCOLS + tid; t < NB; t += S::BLOCK) sB[t] = R(0);
+    for (int t = TILE + NFF * NCOLS + tid; t < NB; t += S::BLOCK) sB[t] = R(0);
   __syncthreads();
   const R *sQ = sW;  // the q_in tile, [el][j][c], until the flux planes are written
 
@@ -392,6 +411,95 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     for (int c = 0; c < NC; ++c) qa[c] = A.a * double(qa_in[c]) + A.b * double(q[c]) + cdt * double(src[c]);
   }
   __syncthreads();
+
+  // ---- 3. contractions ----------------------------------------------------
+  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr in
+  // two products: first the lift over the face fluxes, then -- once the
+  // face fluxes and the q_in tile are no longer read -- the flux planes are
+  // written over them and the volume product accumulates onto the same
+  // registers.  The result goes to [el][i][c] at the start of the array.
+  //
+  // FP32-MFMA: Out[i][col] = sum_k Aop[i][k] B[k][col] with the operator
+  // columns [D (KVP) | Lift (KPAD - KVP)]; V_MFMA_F32_16X16X4F32 operands,
+  // lane l = 16 g + r: A[r][k0 + g], B[k0 + g][col0 + r]; result rows
+  // 4 g + v (v < 4), column col0 + r.
+  constexpr int NTW = (NT + 3) / 4;  // column tiles per wave
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int l = tid & 63, lr = l & 15, lg = l >> 4;
+  mxm_float4 macc[kMFMA ? MT : 1][kMFMA ? NTW : 1];
+  // FP64 on the VALU: the (element, component) columns are the lanes;
+  // each wave group produces IH consecutive output nodes, so the operator
+  // entries are wave-uniform (scalar loads)
+  constexpr int GROUPS = stage_groups(EPB, NC);
+  constexpr int GSIZE = S::BLOCK / GROUPS;
+  constexpr int NPAD = stage_node_pad(EPB, NP, NC);
+  constexpr int IH = NPAD / GROUPS;
+  constexpr int ITER = (NCOL + GSIZE - 1) / GSIZE;  // columns per thread, at most
+  const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
+  const int i0 = grp * IH;
+  const int lane = tid - grp * GSIZE;
+  double dacc[kMFMA ? 1 : ITER][kMFMA ? 1 : IH];
+
+  // one 4-deep step of the MFMA product over operator columns k0.. and
+  // operand rows kb.. of Bb
+  auto kstep = [&](const R *Bb, int k0, int kb) {
+    float a[MT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int ai = (m * 16 + lr) * KPAD + k0 + lg;
+      if constexpr (kStageA)
+        a[m] = sA[ai];
+      else
+        a[m] = A.Aop[ai];
+    }
+#pragma unroll
+    for (int w = 0; w < NTW; ++w) {
+      const int nt = wave * NTW + w;
+      if (nt < NT) {
+        const float bv = float(Bb[(kb + lg) * NCOLS + nt * 16 + lr]);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) macc[m][w] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv, macc[m][w], 0, 0, 0);
+      }
+    }
+  };
+
+  // ---- 3a. lift product over the face fluxes -----------------------------
+  if constexpr (kMFMA) {
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int w = 0; w < NTW; ++w) macc[m][w] = mxm_float4{0.f, 0.f, 0.f, 0.f};
+    if constexpr (kStageA) {
+#pragma unroll
+      for (int k0 = KVP; k0 < KPAD; k0 += 4) kstep(sS, k0, k0 - KVP);
+    } else {
+#pragma unroll 4
+      for (int k0 = KVP; k0 < KPAD; k0 += 4) kstep(sS, k0, k0 - KVP);
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < ITER; ++m) {
+      const int p = lane + m * GSIZE;
+#pragma unroll
+      for (int ii = 0; ii < IH; ++ii) dacc[m][ii] = 0.0;
+      if (p < nE * NC) {
+        const int el = p / NC;
+#pragma unroll 4
+        for (int fr = 0; fr < NFF; ++fr) {
+          const double sv = sS[fr * NCOLS + p];
+          const double *L = A.Lt + fr * NPAD + i0;
+#pragma unroll
+          for (int ii = 0; ii < IH; ++ii) dacc[m][ii] += L[ii] * sv;
+        }
+        const double niv = -sJ[EPB * ND * ND + el];
+#pragma unroll
+        for (int ii = 0; ii < IH; ++ii) dacc[m][ii] *= niv;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- 3b. flux planes over the face fluxes and the q_in tile -------------
   if (node) {
     const int el = tid / NP, j = tid - el * NP;
 #pragma unroll
@@ -399,49 +507,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 #pragma unroll
       for (int c = 0; c < NC; ++c) sW[(k * NP + j) * NCOLS + el * NC + c] = Fk[k * NC + c];
   }
+  if constexpr (kMFMA && KVP > KV)
+    for (int t = KV * NCOLS + tid; t < KVP * NCOLS; t += S::BLOCK) sB[t] = R(0);
   __syncthreads();
 
-  // ---- 3a. contractions ---------------------------------------------------
-  // rhs_i = sum_k sum_j D_k[i][j] F_kj - inv_vol sum_fr Lift[fr][i] S_fr,
-  // written into plane 0 as [el][i][c] after a barrier
+  // ---- 3c. volume product, result as [el][i][c] ---------------------------
   if constexpr (kMFMA) {
-    // Out[i][col] = sum_k Aop[i][k] B[k][col], B = [flux planes ; sS].
-    // V_MFMA_F32_16X16X4F32 operands, lane l = 16 g + r: A[r][k0 + g],
-    // B[k0 + g][col0 + r]; result rows 4 g + v (v < 4), column col0 + r.
-    constexpr int NTW = (NT + 3) / 4;  // column tiles per wave
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int l = tid & 63, lr = l & 15, lg = l >> 4;
-    mxm_float4 acc[MT][NTW];
-#pragma unroll
-    for (int m = 0; m < MT; ++m)
-#pragma unroll
-      for (int w = 0; w < NTW; ++w) acc[m][w] = mxm_float4{0.f, 0.f, 0.f, 0.f};
-    auto kstep = [&](int k0) {
-      float a[MT];
-#pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        const int ai = (m * 16 + lr) * KPAD + k0 + lg;
-        if constexpr (kStageA)
-          a[m] = sA[ai];
-        else
-          a[m] = A.Aop[ai];
-      }
-#pragma unroll
-      for (int w = 0; w < NTW; ++w) {
-        const int nt = wave * NTW + w;
-        if (nt < NT) {
-          const float bv = float(sB[(k0 + lg) * NCOLS + nt * 16 + lr]);
-#pragma unroll
-          for (int m = 0; m < MT; ++m) acc[m][w] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], bv, acc[m][w], 0, 0, 0);
-        }
-      }
-    };
     if constexpr (kStageA) {
 #pragma unroll
-      for (int k0 = 0; k0 < KPAD; k0 += 4) kstep(k0);
+      for (int k0 = 0; k0 < KVP; k0 += 4) kstep(sW, k0, k0);
     } else {
 #pragma unroll 4
-      for (int k0 = 0; k0 < KPAD; k0 += 4) kstep(k0);
+      for (int k0 = 0; k0 < KVP; k0 += 4) kstep(sW, k0, k0);
     }
     __syncthreads();
 #pragma unroll
@@ -454,46 +531,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int i = m * 16 + 4 * lg + v;
-            if (i < NP) sW[(el * NP + i) * NC + c] = R(acc[m][w][v]);
+            if (i < NP) sW[(el * NP + i) * NC + c] = R(macc[m][w][v]);
           }
       }
     }
   } else {
-    // FP64 on the VALU: the (element, component) columns are the lanes;
-    // each wave group produces IH consecutive output nodes, so the operator
-    // entries are wave-uniform (scalar loads)
-    constexpr int GROUPS = stage_groups(EPB, NC);
-    constexpr int GSIZE = S::BLOCK / GROUPS;
-    constexpr int NPAD = stage_node_pad(EPB, NP, NC);
-    constexpr int IH = NPAD / GROUPS;
-    constexpr int ITER = (NCOL + GSIZE - 1) / GSIZE;  // columns per thread, at most
-    const int grp = __builtin_amdgcn_readfirstlane(tid / GSIZE);
-    const int i0 = grp * IH;
-    const int lane = tid - grp * GSIZE;
-    double acc[ITER][IH];
 #pragma unroll
     for (int m = 0; m < ITER; ++m) {
       const int p = lane + m * GSIZE;
-#pragma unroll
-      for (int ii = 0; ii < IH; ++ii) acc[m][ii] = 0.0;
       if (p < nE * NC) {
-        const int el = p / NC;
 #pragma unroll 4
-        for (int fr = 0; fr < NF * NFP; ++fr) {
-          const double sv = sS[fr * NCOLS + p];
-          const double *L = A.Lt + fr * NPAD + i0;
-#pragma unroll
-          for (int ii = 0; ii < IH; ++ii) acc[m][ii] += L[ii] * sv;
-        }
-        const double niv = -sJ[EPB * ND * ND + el];
-#pragma unroll
-        for (int ii = 0; ii < IH; ++ii) acc[m][ii] *= niv;
-#pragma unroll 4
-        for (int kj = 0; kj < ND * NP; ++kj) {
+        for (int kj = 0; kj < KV; ++kj) {
           const double fv = sW[kj * NCOLS + p];
           const double *D = A.Dt + kj * NPAD + i0;
 #pragma unroll
-          for (int ii = 0; ii < IH; ++ii) acc[m][ii] += D[ii] * fv;
+          for (int ii = 0; ii < IH; ++ii) dacc[m][ii] += D[ii] * fv;
         }
       }
     }
@@ -505,7 +557,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
         const int el = p / NC, c = p - el * NC;
 #pragma unroll
         for (int ii = 0; ii < IH; ++ii)
-          if (i0 + ii < NP) sW[(el * NP + i0 + ii) * NC + c] = R(acc[m][ii]);
+          if (i0 + ii < NP) sW[(el * NP + i0 + ii) * NC + c] = R(dacc[m][ii]);
       }
     }
   }
