@@ -28,9 +28,26 @@ template <class R>
 MXM_HD R rmin(R a, R b) { return a < b ? a : b; }
 template <class R>
 MXM_HD R rabs(R a) { return a >= R(0) ? a : -a; }
-// square root in the precision of the argument (host and device)
+// square root in the precision of the argument
 MXM_HD float msqrt(float x) { return sqrtf(x); }
-MXM_HD double msqrt(double x) { return sqrt(x); }
+// FP64: the hardware reciprocal square root refined by Goldschmidt's
+// iteration (g -> sqrt(x), h -> 1/(2 sqrt(x))) and two Newton corrections
+// of g, within an ulp or so of the rounded root.  The fluxes take roots of
+// positive normal numbers or of an exact zero, so the operand scaling of
+// the library square root is not needed; x = 0 (where the reciprocal root
+// is infinite) returns 0.
+__device__ __forceinline__ double msqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  g = fma(d, h, g);
+  return x > 0.0 ? g : 0.0;
+}
 
 // ------------------------------------------------------------ Advection --
 // src/advection.mojo:60-119
