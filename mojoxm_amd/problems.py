@@ -79,6 +79,7 @@ class Problem:
     nsteps: int
     exact: Optional[Callable[[np.ndarray, float], np.ndarray]] = None
     notes: dict = field(default_factory=dict)
+    bcs: Optional[object] = None  # BoundaryConditions / BoundaryConditions2D; None = periodic
 
     @property
     def ndim(self) -> int:
@@ -123,6 +124,102 @@ def taylor_green_3d(N=32, P=3, cfl=0.2, nsteps=200, mach=0.3, flux=ph.FLUX_RUSAN
     L = 2.0 * math.pi
     return Problem(f"taylor_green_3d_n{N}_p{P}", (N, N, N), (L, L, L), P, phys,
                    lambda xyz: taylor_green_ic(xyz, mach=mach, gamma=gamma), dt, nsteps)
+
+
+# ---------------------------------------------------------------------------
+# The 3D drivers of the other trait physics (tetrahedra, P = 2, dt = CFL h /
+# (wave (2P+1)) as in each driver's choose_dt).
+# ---------------------------------------------------------------------------
+def maxwell_cavity_3d(NY=32, P=2, cfl=0.2, t_final=2.0, c=1.0) -> Problem:
+    """examples/maxwell_cavity.mojo:1-80: the lowest standing mode between
+    PEC walls at y = 0 and y = 1, periodic in x and z (NY/2 x NY x 2 cubes,
+    z thickness 2/NY): E_x = sin(pi y) cos(w t), B_z = cos(pi y) sin(w t) / c,
+    w = c pi.  One period T = 2 / c returns the IC."""
+    from .boundary import BoundaryConditions
+    n, L = (NY // 2, NY, 2), (1.0, 1.0, 2.0 / NY)
+    steps, dt = fixed_steps(t_final, apriori_dt(cfl, 1.0 / NY, c, P))
+
+    def exact(xyz, t):
+        y = xyz[..., 1]
+        q = np.zeros(xyz.shape[:-1] + (6,))
+        q[..., 0] = np.sin(math.pi * y) * math.cos(c * math.pi * t)
+        q[..., 5] = np.cos(math.pi * y) * math.sin(c * math.pi * t) / c
+        return q
+    return Problem(f"maxwell_cavity_3d_ny{NY}_p{P}", n, L, P, ph.Maxwell(c=c), lambda xyz: exact(xyz, 0.0), dt, steps,
+                   exact=exact, bcs=BoundaryConditions(0, 0, 1, 1, 0, 0))
+
+
+def mhd_alfven_3d(NX=32, P=2, cfl=0.2, t_final=1.0, amp=0.1, gamma=5.0 / 3.0, rho0=1.0, B0=1.0, p0=0.1, c_h=1.5,
+                  alpha_d=0.5) -> Problem:
+    """examples/mhd_alfven.mojo:1-133: a transverse Alfven wave along +x on
+    the periodic box NX x 4 x 4 cubes of side 1/NX, B = (B0, 0, 0),
+    u_y = A sin(k x - w t), B_y = -A sin(k x - w t), w = k B0 / sqrt(rho0),
+    k = 2 pi; exact for the linearised equations (the finite amplitude adds
+    an O(A^2) magnetic-pressure disturbance)."""
+    n, L = (NX, 4, 4), (1.0, 4.0 / NX, 4.0 / NX)
+    cf = math.sqrt(gamma * p0 / rho0 + (B0 * B0 + amp * amp) / rho0)
+    steps, dt = fixed_steps(t_final, apriori_dt(cfl, 1.0 / NX, max(cf, c_h), P))
+    cA, k = B0 / math.sqrt(rho0), 2.0 * math.pi
+
+    def exact(xyz, t):
+        s = amp * np.sin(k * xyz[..., 0] - cA * k * t)
+        q = np.zeros(xyz.shape[:-1] + (9,))
+        q[..., 0] = rho0
+        q[..., 2] = rho0 * s
+        q[..., 5] = B0
+        q[..., 6] = -s
+        q[..., 4] = p0 / (gamma - 1.0) + 0.5 * rho0 * s * s + 0.5 * (B0 * B0 + s * s)
+        return q
+    phys = ph.IdealMHD(gamma=gamma, min_density=1e-6, min_pressure=1e-6, c_h=c_h, alpha_d=alpha_d)
+    return Problem(f"mhd_alfven_3d_nx{NX}_p{P}", n, L, P, phys, lambda xyz: exact(xyz, 0.0), dt, steps, exact=exact)
+
+
+def langmuir_3d(NX=16, P=2, cfl=0.2, t_final=math.pi, u0=0.01, n0=1.0, m_i=25.0, c=10.0, c_h=12.0, p0=0.01) -> Problem:
+    """examples/two_fluid_langmuir.mojo:1-143: a uniform two-fluid plasma
+    (q_e = -1, m_e = 1, q_i = 1, eps0 = 1) on the periodic box NX x 2 x 2
+    cubes of side 1/NX, electrons started at u_e = (u0, 0, 0).  Uniform in
+    space, so the fluxes cancel and the sources drive a Langmuir
+    oscillation at w^2 = n0 (q_e^2/m_e + q_i^2/m_i) / eps0:
+    E_x = (u0 / w) sin(w t), u_e = u0 - (u0 / w^2)(1 - cos w t),
+    u_i = (u0 / (m_i w^2))(1 - cos w t)."""
+    gam = 5.0 / 3.0
+    n, L = (NX, 2, 2), (1.0, 2.0 / NX, 2.0 / NX)
+    steps, dt = fixed_steps(t_final, apriori_dt(cfl, 1.0 / NX, max(c, c_h), P))
+    w = math.sqrt(n0 * (1.0 + 1.0 / m_i))
+    phys = ph.FiveMomentTwoFluid(gamma_e=gam, gamma_i=gam, q_e=-1.0, m_e=1.0, q_i=1.0, m_i=m_i, eps0=1.0,
+                                 c_light=c, c_h=c_h, alpha_d=0.5, min_density=1e-6, min_pressure=1e-8)
+
+    def ic(xyz):
+        q = np.zeros(xyz.shape[:-1] + (17,))
+        q[..., 0], q[..., 1] = n0, n0 * u0
+        q[..., 4] = p0 / (gam - 1.0) + 0.5 * n0 * u0 * u0
+        q[..., 5], q[..., 9] = m_i * n0, p0 / (gam - 1.0)
+        return q
+
+    def exact_fields(t):
+        """(E_x, u_e, u_i) at time t."""
+        return (u0 / w * math.sin(w * t), u0 - u0 / (w * w) * (1.0 - math.cos(w * t)),
+                u0 / (m_i * w * w) * (1.0 - math.cos(w * t)))
+    return Problem(f"langmuir_3d_nx{NX}_p{P}", n, L, P, phys, ic, dt, steps,
+                   notes={"omega": w, "exact_fields": exact_fields})
+
+
+def shallow_water_drop_3d(N=32, P=2, cfl=0.2, t_final=1.0, g=1.0, h0=1.0, amp=0.2, sigma=0.07) -> Problem:
+    """examples/shallow_water_drop.mojo:1-97: a Gaussian bump
+    h = h0 + a exp(-r^2 / sigma^2) at the centre of the unit square basin
+    with slip walls on x and y, periodic in z (N x N x 2 cubes, z thickness
+    2/N), at rest.  Total mass is conserved."""
+    from .boundary import BoundaryConditions
+    n, L = (N, N, 2), (1.0, 1.0, 2.0 / N)
+    steps, dt = fixed_steps(t_final, apriori_dt(cfl, 1.0 / N, math.sqrt(g * (h0 + amp)), P))
+
+    def ic(xyz):
+        r2 = (xyz[..., 0] - 0.5) ** 2 + (xyz[..., 1] - 0.5) ** 2
+        q = np.zeros(xyz.shape[:-1] + (3,))
+        q[..., 0] = h0 + amp * np.exp(-r2 / (sigma * sigma))
+        return q
+    return Problem(f"shallow_water_drop_3d_n{N}_p{P}", n, L, P, ph.ShallowWater(g=g, h_min=1e-4), ic, dt, steps,
+                   bcs=BoundaryConditions(1, 1, 1, 1, 0, 0))
 
 
 # ---------------------------------------------------------------------------

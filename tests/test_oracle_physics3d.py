@@ -305,3 +305,33 @@ def test_twofluid_speed_is_largest_signal():
         tf = ph.FiveMomentTwoFluid(gamma_e=5.0 / 3.0, gamma_i=1.4, c_light=cl, c_h=ch)
         _, sp = O.numerical_flux(tf.phys_id, tf.params(), q, q, n)
         assert sp == pytest.approx(want, rel=1e-14)
+
+
+def test_langmuir_uniform_plasma_exact():
+    """examples/two_fluid_langmuir.mojo: a uniform plasma has no flux
+    divergence, so the stage is the source ODE; its (E_x, u_e, u_i) follow
+    the Langmuir oscillation at w^2 = n0 (1 + m_e / m_i) in closed form
+    (SSPRK3's O(dt^3) error is far below the tolerance here)."""
+    from mojoxm_amd import problems as pr
+    p = pr.langmuir_3d(t_final=0.2)
+    m = O.Mesh(3, p.P, p.n, p.L)
+    out = m.step(p.physics.phys_id, p.physics.params(), p.ic(m.elem_node_xyz), p.dt, p.nsteps)
+    E, ue, ui = p.notes["exact_fields"](p.t_final)
+    assert out[..., 10].mean() == pytest.approx(E, rel=1e-10)
+    assert (out[..., 1] / out[..., 0]).mean() == pytest.approx(ue, rel=1e-10)
+    assert (out[..., 6] / out[..., 5]).mean() == pytest.approx(ui, rel=1e-8)
+    assert out[..., 10].std() < 1e-15
+
+
+def test_maxwell_cavity_with_pec_walls_converges():
+    """examples/maxwell_cavity.mojo: the standing mode between PEC walls at
+    y = 0, 1 converges at the P = 2 rate (error / 8 per halving of h)."""
+    from mojoxm_amd import problems as pr
+    errs = []
+    for NY in (8, 16):
+        p = pr.maxwell_cavity_3d(NY=NY, t_final=0.5)
+        m = O.Mesh(3, p.P, p.n, p.L, [0, 0, 1, 1, 0, 0])
+        out = m.step(p.physics.phys_id, p.physics.params(), p.ic(m.elem_node_xyz), p.dt, p.nsteps)
+        ex = p.exact(m.elem_node_xyz, p.t_final)
+        errs.append(np.linalg.norm(out - ex) / np.linalg.norm(ex))
+    assert errs[1] < errs[0] / 6 and errs[1] < 2e-4
