@@ -221,6 +221,16 @@ int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us
  * (src/diagnostics.mojo:161-223), whose unweighted nodal mean (:170) is
  * replaced by the mass-weighted integral. */
 int mxm_dg_diagnostics(void *solver, double *out, int cap, int *n);
+/* Largest signal speed over the nodes of the current state, reduced on the
+ * device (FP64): |u| + c for the Euler systems, |u| + sqrt(g h) for shallow
+ * water, max(|u| + fast speed across B, c_h) for MHD, c for Maxwell, the
+ * largest of the two species' |u| + c, c and c_h for the two-fluid system,
+ * |v| for advection; NaN if the state holds a NaN.  A caller sets
+ * dt = CFL h / (speed (2P + 1)).  No reference counterpart: the
+ * reference's drivers use an a-priori speed (examples/euler_vortex_2d_gpu.
+ * mojo:148-154) and discard the wave speeds its flux hooks return
+ * (src/solver.mojo:229,266,268). */
+int mxm_dg_max_wave_speed(void *solver, double *speed);
 /* Launch configurations (no reference counterpart; the reference has one
  * launch shape per kernel).  Each configuration of the stage kernel is an
  * occupancy target (waves per SIMD) that bounds its register allocation;
@@ -276,6 +286,8 @@ int mxm_batch_status(void *batch, int id, int *steps_done);
 int mxm_batch_get_state(void *batch, int id, double *q, long long n);
 /* mxm_dg_diagnostics of one instance. */
 int mxm_batch_diagnostics(void *batch, int id, double *out, int cap, int *n);
+/* mxm_dg_max_wave_speed of one instance. */
+int mxm_batch_max_wave_speed(void *batch, int id, double *speed);
 /* Drop an instance; its device memory is reclaimed at the next step. */
 int mxm_batch_remove(void *batch, int id);
 /* Device bytes held by the batch. */

@@ -74,10 +74,12 @@ EXPORTS = {
                         C.c_int),
     "mxm_dg_launch_config": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_diagnostics": ([C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mxm_dg_max_wave_speed": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "mxm_dg_destroy": ([C.c_void_p], C.c_int),
     "mxm_batch_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mxm_batch_autotune": ([C.c_void_p, C.c_int], C.c_int),
     "mxm_batch_diagnostics": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mxm_batch_max_wave_speed": ([C.c_void_p, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "mxm_batch_add_2d": ([C.c_void_p, C.POINTER(Mesh2DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,
                           C.c_int, C.c_double, C.c_void_p, C.c_longlong, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_add_3d": ([C.c_void_p, C.POINTER(Mesh3DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,

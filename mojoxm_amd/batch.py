@@ -117,6 +117,12 @@ class Batch:
         L.check(L.lib().mxm_batch_diagnostics(self._h, ident, out, 3 * nc + 2, C.byref(n)), "mxm_batch_diagnostics")
         return Diagnostics.from_flat(list(out), nc)
 
+    def max_wave_speed(self, ident: int) -> float:
+        """mxm_dg_max_wave_speed of one instance."""
+        v = C.c_double()
+        L.check(L.lib().mxm_batch_max_wave_speed(self._h, ident, C.byref(v)), "mxm_batch_max_wave_speed")
+        return v.value
+
     def remove(self, ident: int) -> None:
         L.check(L.lib().mxm_batch_remove(self._h, ident), "mxm_batch_remove")
         self._shape.pop(ident, None)

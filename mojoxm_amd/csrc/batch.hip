@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/mxm_dg.h"
+#include "cfl.hpp"
 #include "diag.hpp"
 #include "limiter.hpp"
 #include "runtime.hpp"
@@ -519,6 +520,35 @@ struct Batch {
       read_state(g, g.qf, in, h);
   }
 
+  // the largest signal speed over the nodes of one instance
+  double max_wave_speed(int id) {
+    BInst &in = inst(id);
+    BGroup &g = group_with(id);
+    if (g.dirty) repack(g);
+    const int nt = in.t.til.ntiles;
+    if (g.dpart.n < (size_t)nt) g.dpart.alloc(nt);
+    if (g.dout.n < 1) g.dout.alloc(1);
+    auto fill = [&](auto &a) {
+      a.prm = g.iprm.p, a.tile_inst = g.tile_inst.p, a.tile_count = g.tile_count.p;
+      a.epb = g.epb, a.NP = g.NP, a.tile0 = in.base_tile, a.ntiles = nt, a.part = g.dpart.p, a.out = g.dout.p;
+    };
+    if (g.prec == MXM_PREC_FP64) {
+      SpeedArgs<double> a{};
+      a.q = g.qd[0].p;
+      fill(a);
+      MXM_HIP(launch_max_speed(g.phys, true, &a, stream));
+    } else {
+      SpeedArgs<float> a{};
+      a.q = g.qf[0].p;
+      fill(a);
+      MXM_HIP(launch_max_speed(g.phys, false, &a, stream));
+    }
+    double h = 0.0;
+    MXM_HIP(hipMemcpyAsync(&h, g.dout.p, sizeof(double), hipMemcpyDeviceToHost, stream));
+    MXM_HIP(hipStreamSynchronize(stream));
+    return h;
+  }
+
   // diagnostics of one instance (Solver::diagnostics over its tiles)
   void diagnostics(int id, double *h) {
     BInst &in = inst(id);
@@ -688,6 +718,14 @@ int mxm_batch_status(void *h, int id, int *steps_done) {
 int mxm_batch_get_state(void *h, int id, double *q, long long n) {
   return guarded([&] {
     as_batch(h)->get_state(id, q, n);
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_max_wave_speed(void *h, int id, double *speed) {
+  return guarded([&] {
+    if (!speed) throw ArgError("null output");
+    *speed = as_batch(h)->max_wave_speed(id);
     return MXM_OK;
   });
 }

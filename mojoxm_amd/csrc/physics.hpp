@@ -30,6 +30,10 @@ template <class R>
 MXM_HD R rabs(R a) { return a >= R(0) ? a : -a; }
 // square root in the precision of the argument
 MXM_HD float msqrt(float x) { return sqrtf(x); }
+// max that keeps a NaN in its first argument (the CFL reduction must not
+// hide a broken state behind a constant speed)
+template <class R>
+MXM_HD R nmax(R a, R b) { return (a > b || a != a) ? a : b; }
 // FP64: the hardware reciprocal square root refined by Goldschmidt's
 // iteration (g -> sqrt(x), h -> 1/(2 sqrt(x))) and two Newton corrections
 // of g, within an ulp or so of the rounded root.  The fluxes take roots of
@@ -72,6 +76,8 @@ struct Advection {
   // (src/advection.mojo:98-112)
   MXM_HD void ghost(const R *, int bc, const R *, R *g) const { g[0] = bc == BC_INFLOW ? inflow : R(0); }
   MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
+  // largest signal speed at a node, over all directions (CFL reduction)
+  MXM_HD R max_speed(const R *) const { return msqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -358,6 +364,11 @@ struct Euler {
   // NaN-safe positivity rescue, src/euler.mojo:758-792
   // Evaluated in FP64 whatever the state precision (the stage kernels pass
   // an FP64 copy of the combined node).
+  // |u| + c (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    const Prim s = prim(q);
+    return msqrt(s.u[0] * s.u[0] + s.u[1] * s.u[1] + s.u[2] * s.u[2]) + msqrt(gamma * s.p * s.ir);
+  }
   MXM_HD void limit(double *q) const {
     const double e0 = pmin_d / (gamma_d - 1.0);
     if (!(q[0] > rmin_d)) {
@@ -471,6 +482,12 @@ struct Euler2D {
     R r = q[0] < rmin_ ? rmin_ : q[0];
     s[0] = R(0), s[1] = r * gx, s[2] = r * gy, s[3] = q[1] * gx + q[2] * gy;
   }
+  // |u| + c (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    R r, u, v, p;
+    prim(q, r, u, v, p);
+    return msqrt(u * u + v * v) + msqrt(gamma * p / r);
+  }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -530,6 +547,11 @@ struct SW2D {
     g[2] = in ? inflow[2] : (wall ? q[2] - R(2) * mn * n[1] : q[2]);
   }
   MXM_HD void source(const R *, R *s) const { s[0] = s[1] = s[2] = R(0); }
+  // |u| + sqrt(g h) (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    const R h = q[0] < hmin ? hmin : q[0];
+    return msqrt(q[1] * q[1] + q[2] * q[2]) / h + msqrt(grav * h);
+  }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -593,6 +615,14 @@ struct MHDGLM2D {
   MXM_HD void source(const R *, R *s) const {
     for (int c = 0; c < 7; ++c) s[c] = R(0);
   }
+  // max(|u| + the fast speed across B, c_h) (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    const R r = q[0] < rmin_ ? rmin_ : q[0];
+    const R BB = q[3] * q[3] + q[4] * q[4];
+    R p = (gamma - R(1)) * (q[5] - R(0.5) * (q[1] * q[1] + q[2] * q[2]) / r - R(0.5) * BB);
+    if (p < pmin_) p = pmin_;
+    return nmax(msqrt(q[1] * q[1] + q[2] * q[2]) / r + msqrt((gamma * p + BB) / r), ch);
+  }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -644,6 +674,8 @@ struct Maxwell2D {
     R c2 = c * c;
     for (int d = 0; d < 3; ++d) s[d] = -(c2 * J[d]), s[3 + d] = -M[d];
   }
+  // the light speed (CFL reduction)
+  MXM_HD R max_speed(const R *) const { return rabs(c); }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -669,6 +701,8 @@ struct Advection2D {
   // (src/local_mesh_2d_gpu_advection.mojo:285-298)
   MXM_HD void ghost(const R *, int bc, const R *, R *g) const { g[0] = bc == BC_INFLOW ? inflow : R(0); }
   MXM_HD void source(const R *, R *s) const { s[0] = R(0); }
+  // |v| (CFL reduction)
+  MXM_HD R max_speed(const R *) const { return msqrt(vx * vx + vy * vy); }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -725,6 +759,11 @@ struct ShallowWater3 {
     if (bc == BC_WALL) reflect3(q, n, 1, 2, gs);
   }
   MXM_HD void source(const R *, R *s) const { s[0] = s[1] = s[2] = R(0); }
+  // |u| + sqrt(g h) (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    const R h = q[0] > hmin ? q[0] : hmin;
+    return msqrt(q[1] * q[1] + q[2] * q[2]) / h + msqrt(g * h);
+  }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -780,6 +819,8 @@ struct Maxwell3 {
     const R c2 = c * c;
     for (int d = 0; d < 3; ++d) s[d] = -(c2 * J[d]), s[3 + d] = -M[d];
   }
+  // the light speed (CFL reduction)
+  MXM_HD R max_speed(const R *) const { return rabs(c); }
   template <class T>
   MXM_HD void limit(T *) const {}
 };
@@ -856,6 +897,12 @@ struct IdealMHD3 {
   MXM_HD void source(const R *q, R *s) const {
     for (int k = 0; k < 8; ++k) s[k] = R(0);
     s[8] = -alpha * q[8];
+  }
+  // max(|u| + the fast speed across B, c_h) (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    const R rho = q[0] > rmin ? q[0] : rmin, ir = R(1) / rho;
+    const R b2 = q[5] * q[5] + q[6] * q[6] + q[7] * q[7];
+    return nmax(msqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) * ir + msqrt((gamma * pres(q, rho, ir) + b2) * ir), ch);
   }
   template <class T>
   MXM_HD void limit(T *) const {}
@@ -954,6 +1001,16 @@ struct TwoFluid3 {
     s[4] = qme * re * (ue[0] * E[0] + ue[1] * E[1] + ue[2] * E[2]);
     s[9] = qmi * ri * (ui[0] * E[0] + ui[1] * E[1] + ui[2] * E[2]);
     s[16] = -alpha * q[16];
+  }
+  // max over the species of |u| + c_s, c and c_h (CFL reduction)
+  MXM_HD R max_speed(const R *q) const {
+    R sp[2];
+    for (int k = 0; k < 2; ++k) {
+      const R *qs = q + 5 * k;
+      const R gm = k ? gi : ge, rho = qs[0] > rmin ? qs[0] : rmin, ir = R(1) / rho;
+      sp[k] = msqrt(qs[1] * qs[1] + qs[2] * qs[2] + qs[3] * qs[3]) * ir + msqrt(gm * species_p(qs, gm, ir) * ir);
+    }
+    return nmax(sp[0], nmax(sp[1], rmax(rabs(cl), ch)));
   }
   template <class T>
   MXM_HD void limit(T *) const {}

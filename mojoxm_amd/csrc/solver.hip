@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/mxm_dg.h"
+#include "cfl.hpp"
 #include "diag.hpp"
 #include "host.hpp"
 #include "limiter.hpp"
@@ -207,6 +208,31 @@ struct Solver {
     MXM_HIP(hipMemcpyAsync(h, dout.p, (3 * NC + 2) * sizeof(double), hipMemcpyDeviceToHost, stream));
     MXM_HIP(hipStreamSynchronize(stream));
     if (!diag_nvel(phys)) h[3 * NC] = h[3 * NC + 1] = std::nan("");
+  }
+
+  // the largest signal speed over the nodes of the current state
+  double max_wave_speed() {
+    if (dpart.n < (size_t)ntiles) dpart.alloc(ntiles);
+    if (dout.n < 1) dout.alloc(1);
+    auto fill = [&](auto &a) {
+      a.prm = dprm.p, a.tile_inst = nullptr, a.tile_count = tile_count.p;
+      a.epb = til.epb, a.NP = NP, a.tile0 = 0, a.ntiles = ntiles, a.part = dpart.p, a.out = dout.p;
+    };
+    if (prec == MXM_PREC_FP64) {
+      SpeedArgs<double> a{};
+      a.q = q.p;
+      fill(a);
+      MXM_HIP(launch_max_speed(phys, true, &a, stream));
+    } else {
+      SpeedArgs<float> a{};
+      a.q = f.p;
+      fill(a);
+      MXM_HIP(launch_max_speed(phys, false, &a, stream));
+    }
+    double h = 0.0;
+    MXM_HIP(hipMemcpyAsync(&h, dout.p, sizeof(double), hipMemcpyDeviceToHost, stream));
+    MXM_HIP(hipStreamSynchronize(stream));
+    return h;
   }
 
   void step(double dt, int n) {
@@ -657,6 +683,15 @@ int mxm_dg_diagnostics(void *h, double *out, int cap, int *n) {
     if (n) *n = nv;
     if (!out || cap < nv) throw ArgError("diagnostics need 3*NC + 2 values");
     s->diagnostics(out);
+    return MXM_OK;
+  });
+}
+
+int mxm_dg_max_wave_speed(void *h, double *speed) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    if (!speed) throw ArgError("null output");
+    *speed = s->max_wave_speed();
     return MXM_OK;
   });
 }

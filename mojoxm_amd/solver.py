@@ -144,6 +144,14 @@ class Solver:
         L.check(L.lib().mxm_dg_diagnostics(self._h, out, 3 * nc + 2, C.byref(n)), "mxm_dg_diagnostics")
         return Diagnostics.from_flat(list(out), nc)
 
+    def max_wave_speed(self) -> float:
+        """Largest signal speed over the nodes of the current state, reduced
+        on the device (mxm_dg_max_wave_speed); dt = CFL h / (speed (2P+1))
+        is time_integrator.cfl_dt."""
+        v = C.c_double()
+        L.check(L.lib().mxm_dg_max_wave_speed(self._h, C.byref(v)), "mxm_dg_max_wave_speed")
+        return v.value
+
     def autotune(self, dt: float, reps: int = 20):
         """Time every launch configuration of the stage kernel on this mesh
         (stage 1 of SSPRK3, which leaves the state unchanged) and keep the

@@ -22,6 +22,13 @@ def apriori_dt(cfl: float, h: float, wave: float, P: int) -> float:
     return cfl * h / (wave * (2 * P + 1))
 
 
+def cfl_dt(solver, cfl: float, h: float) -> float:
+    """dt = CFL h / (s (2P+1)) with s the largest signal speed of the
+    solver's current state, reduced on the device (Solver.max_wave_speed):
+    the measured counterpart of apriori_dt."""
+    return apriori_dt(cfl, h, solver.max_wave_speed(), solver.P)
+
+
 def fixed_steps(t_final: float, dt_est: float):
     """steps = ceil(T/dt), dt = T/steps (the config convention)."""
     n = max(1, int(math.ceil(t_final / dt_est - 1e-12)))

@@ -136,6 +136,10 @@ int orc_nodal_deriv(int ndim, int P, double *Dn);
 /* Domain diagnostics of q[K][NP][NC] (diag.c): out[3*NC + 2] = mass-weighted
  * int q_c dV, int q_c^2 dV, max |q_c|, then KE and enstrophy (NaN for
  * physics without a velocity). */
+/* Largest signal speed of one node state over all directions (cfl.c);
+ * NaN for an unknown physics or a NaN state. */
+double orc_max_speed(int phys, const double *prm, const double *q);
+
 int orc_diagnostics(const orc_mesh_t *m, const double *w, const double *Dn, int phys, const double *q,
                     double *out);
 /* OpenMP thread count used by the stage loops (0 = runtime default). */

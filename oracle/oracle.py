@@ -79,6 +79,8 @@ def lib():
         L.orc_nodal_deriv.argtypes = [C.c_int, C.c_int, _dp]
         L.orc3_source.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_diagnostics.argtypes = [C.POINTER(_Mesh), _dp, _dp, C.c_int, _dp, _dp]
+        L.orc_max_speed.argtypes = [C.c_int, _dp, _dp]
+        L.orc_max_speed.restype = C.c_double
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -270,6 +272,20 @@ def internal_flux(phys, prm, q):
     if lib().orc_internal_flux(phys, _prm(prm), _f(q), f):
         raise RuntimeError("internal_flux")
     return f.reshape(3, nc)
+
+
+def max_speed(phys, prm, q):
+    """Largest signal speed of one node state (cfl.c)."""
+    return lib().orc_max_speed(phys, _prm(prm), _f(q))
+
+
+def max_wave_speed(phys, prm, q):
+    """Largest signal speed over every node of a state [K, NP, NC]; NaN if
+    any node's is."""
+    q = np.asarray(q, dtype=np.float64)
+    nc = q.shape[-1]
+    sp = [max_speed(phys, prm, v) for v in q.reshape(-1, nc)]
+    return float(np.nan if np.isnan(sp).any() else max(sp))
 
 
 def source(phys, prm, q):
