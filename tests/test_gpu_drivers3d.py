@@ -110,3 +110,23 @@ def test_shallow_water_drop_conserves_mass(precision):
     print(m0, m1, flush=True)
     assert abs(m1 / m0 - 1.0) < (1e-12 if precision == M.PREC_FP64 else 1e-6)
     assert q[..., 0].min() > 0.9 and q[..., 0].max() < q0[..., 0].max() - 0.05
+
+
+def test_frame_writer_writes_device_state(tmp_path):
+    """mojoxm_amd.vtu.FrameWriter frames of a running Solver read back as
+    the state's components (Float32)."""
+    from mojoxm_amd import vtu
+    p = pr.maxwell_cavity_3d(NY=8, t_final=0.1)
+    mesh = M.LocalMesh(*p.n, *p.L, P=p.P)
+    with M.Solver(p.n, p.L, p.physics, P=p.P, bcs=p.bcs) as s:
+        s.set_state(p.ic(mesh.elem_node_xyz))
+        fw = vtu.FrameWriter(s, mesh.elem_node_xyz, str(tmp_path), components={"Ex": 0, "Bz": 5})
+        fw.write(0.0)
+        s.step_ssprk3(p.dt, p.nsteps)
+        path = fw.write(p.t_final)
+        q = s.get_state()
+        pvd = fw.finalize()
+    r = vtu.read_vtu(path)
+    assert np.array_equal(r["fields"]["Bz"], q[..., 5].astype(np.float32).ravel())
+    assert np.abs(r["fields"]["Bz"]).max() > 0.0
+    assert open(pvd).read().count("<DataSet") == 2
