@@ -6,8 +6,10 @@
 // The face-node maps are derived geometrically: every Lagrange node of the
 // grid lies on the integer lattice  X = P*cell + sum_v a_v * corner_v  (in
 // units of h/P), so the node a neighbour sees at a canonical face slot is
-// found by matching lattice points modulo the periodic box, and the slot
-// permutation of each element follows from its own face-local node map.
+// found by matching lattice points, with the neighbour's cell taken across
+// a periodic boundary without wrapping (exact, also with one cell per
+// period), and the slot permutation of each element follows from its own
+// face-local node map.
 // Face normals and areas come from barycentric gradients of the side-0
 // element (outward normal of face f is -grad(lambda_f)/|grad(lambda_f)|).
 //
@@ -68,16 +70,6 @@ std::array<int, 3> canon_slot(int m, int P) {
       if (idx == m) return {b0, b1, P - b0 - b1};
   throw std::runtime_error("mesh: canonical slot out of range");
 }
-
-struct Lattice3 {
-  int N[3];
-  int P;
-  bool same(std::array<int, 3> a, std::array<int, 3> b) const {
-    for (int d = 0; d < 3; ++d)
-      if (wrap(a[d] - b[d], P * N[d]) != 0) return false;
-    return true;
-  }
-};
 
 }  // namespace
 
@@ -155,7 +147,6 @@ HostMesh build_mesh_3d(int Nx, int Ny, int Nz, double lx, double ly, double lz, 
     }
   }
 
-  Lattice3 lat{{Nx, Ny, Nz}, P};
   auto node_lattice = [&](int cube_i, int cube_j, int cube_k, int t, int n) {
     std::array<int, 3> X{P * cube_i, P * cube_j, P * cube_k};
     for (int v = 0; v < 4; ++v) {
@@ -198,6 +189,12 @@ HostMesh build_mesh_3d(int Nx, int Ny, int Nz, double lx, double ly, double lz, 
       int lf = -1;
       for (int q = 0; q < 4; ++q)
         if (m.elem_faces[e * 4 + q] == fid && m.elem_face_side[e * 4 + q] == s) lf = q;
+      // the element's cube relative to the owner, unwrapped, so that the
+      // lattice points compare exactly (also with one cube per period)
+      const int *inf = TFI[t][lf];
+      const int ui = oi - inf[1], uj = oj - inf[2], uk = ok - inf[3];
+      if (wrap(ui, Nx) != ei || wrap(uj, Ny) != ej || wrap(uk, Nz) != ek)
+        throw std::runtime_error("mesh: face owner offset mismatch");
       for (int slot = 0; slot < NFP; ++slot) {
         auto b = canon_slot(slot, P);
         std::array<int, 3> Xm{P * oi, P * oj, P * ok};
@@ -208,7 +205,7 @@ HostMesh build_mesh_3d(int Nx, int Ny, int Nz, double lx, double ly, double lz, 
         int found = -1, rslot = -1;
         for (int r = 0; r < NFP; ++r) {
           int n = re.fmap[lf * NFP + r];
-          if (lat.same(node_lattice(ei, ej, ek, t, n), Xm)) {
+          if (node_lattice(ui, uj, uk, t, n) == Xm) {
             found = n;
             rslot = r;
             break;
@@ -376,12 +373,14 @@ HostMesh build_mesh_2d(int Nx, int Ny, double lx, double ly, const int *bc_in, i
                                 P * j + P * va[1] + slot * (vb[1] - va[1])};
           for (int s = 0; s < 2; ++s) {
             const int e = s ? e1 : e0, lf = s ? l1 : l0, t = s ? t1 : t0;
-            const int ci = s ? ni : i, cj = s ? nj : j;
+            // the neighbour's cell unwrapped, so that the lattice points
+            // compare exactly (also with one cell per period)
+            const int ci = s ? i + FT[ft][4] : i, cj = s ? j + FT[ft][5] : j;
             int found = -1, rslot = -1;
             for (int r = 0; r < NE; ++r) {
               int n = re.fmap[lf * NE + r];
               auto X = lattice(ci, cj, t, n);
-              if (wrap(X[0] - Xm[0], P * Nx) == 0 && wrap(X[1] - Xm[1], P * Ny) == 0) {
+              if (X[0] == Xm[0] && X[1] == Xm[1]) {
                 found = n;
                 rslot = r;
                 break;

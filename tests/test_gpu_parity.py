@@ -221,3 +221,15 @@ def test_config1_full_run_parity(oracle):
     assert rel_l2(got, want) < TOL
     err = rel_l2(got, q0)
     assert err < 0.1  # DG P=2 on 16^3 damps the sigma=0.1 pulse by a few percent
+
+
+@pytest.mark.parametrize("n", [(1, 2, 2), (2, 1, 2), (2, 2, 1), (1, 1, 1), (1, 3), (4, 1), (1, 1)])
+@pytest.mark.parametrize("P", [1, 3])
+def test_one_cell_per_period(oracle, n, P):
+    """A periodic axis of one cell: the two sides of a face are the same
+    cell's opposite faces, in the same tile."""
+    L = (1.0, 0.8, 1.2)[: len(n)]
+    phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
+    bc = [0, 0, 1, 1, 0, 0][: 2 * len(n)]
+    got, want, _ = run_pair(oracle, n, L, P, phys, bc, nsteps=2, seed=P + len(n))
+    assert rel_l2(got, want) < TOL

@@ -139,3 +139,29 @@ def test_config_sizes(oracle):
     assert (m.num_elements, m.num_faces) == (24576, 49152)
     m2 = M.LocalMesh2D(128, 128, 10.0, 10.0, P=3)
     assert (m2.num_elements, m2.num_faces) == (32768, 49152)
+
+
+@pytest.mark.parametrize("n", [(1, 2, 2), (2, 1, 2), (2, 2, 1), (1, 1, 1), (3, 1, 2)])
+@pytest.mark.parametrize("bc", [[0] * 6, [1, 2, 0, 0, 0, 0], [0, 0, 0, 0, 1, 1]])
+def test_library_mesh3d_one_cell_per_period(oracle, n, bc):
+    """A periodic axis of one cube: a face's two sides are the same cube's
+    opposite faces.  The library's tables equal the oracle's.  (The
+    centroid-direction invariant is not defined here: across a one-cube
+    period the two centroids are half a period apart either way.)"""
+    L = (1.0, 2.0, 1.5)
+    for P in (1, 2, 3):
+        pm = M.LocalMesh(*n, *L, bcs=M.BoundaryConditions(*bc), P=P)
+        om = oracle.Mesh(3, P, n, L, bc)
+        for k in INT_TABLES:
+            assert (getattr(pm, k) == getattr(om, k)).all(), (P, k)
+
+
+@pytest.mark.parametrize("n", [(1, 3), (4, 1), (1, 1)])
+@pytest.mark.parametrize("bc", [[0] * 4, [1, 2, 0, 0], [0, 0, 3, 1]])
+def test_library_mesh2d_one_cell_per_period(oracle, n, bc):
+    L = (2.0, 1.5)
+    for P in (1, 2, 3):
+        pm = M.LocalMesh2D(*n, *L, bcs=M.BoundaryConditions2D(*bc), P=P)
+        om = oracle.Mesh(2, P, n, L, bc)
+        for k in INT_TABLES:
+            assert (getattr(pm, k) == getattr(om, k)).all(), (P, k)
