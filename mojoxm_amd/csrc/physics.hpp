@@ -591,14 +591,35 @@ struct MHDGLM2D {
     if (disc < R(0)) disc = R(0);
     speed = rmax(msqrt(u * u + v * v) + msqrt(R(0.5) * (s + msqrt(disc))), ch);
   }
+  // F.n of one side (the x and y fluxes of state() dotted with n) and its
+  // Rusanov speed; 7 values live instead of 14
+  MXM_HD void side(const R *q, R nx, R ny, R *Fn, R &speed) const {
+    R r = q[0] < rmin_ ? rmin_ : q[0];
+    R u = q[1] / r, v = q[2] / r, Bx = q[3], By = q[4], E = q[5], psi = q[6];
+    R BB = Bx * Bx + By * By;
+    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2]) / r;
+    R p = (gamma - R(1)) * (E - ke - R(0.5) * BB);
+    if (p < pmin_) p = pmin_;
+    R ps = p + R(0.5) * BB, ub = u * Bx + v * By;
+    Fn[0] = q[1] * nx + q[2] * ny;
+    Fn[1] = (q[1] * u + ps - Bx * Bx) * nx + (q[2] * u - By * Bx) * ny;
+    Fn[2] = (q[1] * v - Bx * By) * nx + (q[2] * v + ps - By * By) * ny;
+    Fn[3] = psi * nx + (v * Bx - u * By) * ny;
+    Fn[4] = (u * By - v * Bx) * nx + psi * ny;
+    Fn[5] = ((E + ps) * u - Bx * ub) * nx + ((E + ps) * v - By * ub) * ny;
+    Fn[6] = ch * ch * (Bx * nx + By * ny);
+    R cs2 = gamma * p / r, ca2 = BB / r, s = cs2 + ca2;
+    R disc = s * s - R(4) * cs2 * (Bx * Bx) / r;
+    if (disc < R(0)) disc = R(0);
+    speed = rmax(msqrt(u * u + v * v) + msqrt(R(0.5) * (s + msqrt(disc))), ch);
+  }
   template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
-    R FL[14], FR[14], sl, sr;
-    state(ql, FL, sl);
-    state(qr, FR, sr);
+    R FL[7], FR[7], sl, sr;
+    side(ql, n[0], n[1], FL, sl);
+    side(qr, n[0], n[1], FR, sr);
     R al = sl > sr ? sl : sr;
-    for (int c = 0; c < 7; ++c)
-      f[c] = R(0.5) * ((FL[c] + FR[c]) * n[0] + (FL[7 + c] + FR[7 + c]) * n[1]) - R(0.5) * al * (qr[c] - ql[c]);
+    for (int c = 0; c < 7; ++c) f[c] = R(0.5) * (FL[c] + FR[c]) - R(0.5) * al * (qr[c] - ql[c]);
   }
   // WALL reflects m.n and B.n and flips psi (src/local_mesh_2d_gpu_mhd_glm.mojo:302-312)
   MXM_HD void ghost(const R *q, int bc, const R *n, R *g) const {
