@@ -293,7 +293,7 @@ static int mat_inv_l(const real *m, int n, real *inv) {
 /* Basis coefficients C[i][k] = Vinv[k][i], src/reference.mojo:346-387. */
 static int basis_coefs(const int *nodes, const int *monos, int N, int P,
                        real *C) {
-    real *V = (real *)malloc(sizeof(real) * N * N);
+    real *V = (real *)calloc((size_t)N * N, sizeof(real));
     real *Vi = (real *)malloc(sizeof(real) * N * N);
     real Pf = (real)P;
     for (int j = 0; j < N; ++j) {
