@@ -243,6 +243,31 @@ struct Euler {
       rf[2] = R(0.5) * (rl[2] * ul + rr[2] * ur + cm * (rl[2] - rr[2]));
       rf[3] = R(0.5) * (rl[3] * ul + rr[3] * ur + cm * (rl[3] - rr[3]));
       rf[4] = R(0.5) * ((rl[4] + pl) * ul + (rr[4] + pr) * ur + cm * (rl[4] - rr[4]));
+    } else if constexpr (FLUX == 3 && !efix) {
+      // HLLEC without the entropy fix in its upwind-side form:
+      // F = F_K + s_K^- (U*_K - U_K) on the side K of the contact (s_L^- =
+      // min(s_L, 0) on the left, s_R^+ = max(s_R, 0) on the right).  The
+      // star states satisfy the jump conditions across each of the three
+      // waves, so this equals the wave sum of src/euler.mojo:415-520
+      // (1/2 (F_L + F_R) - 1/2 sum_w |s_w| W_w) up to rounding, with one
+      // star state and one physical flux instead of two.
+      R ra[5];
+      roe_avg(rl, rr, L, Rt, ra);
+      const R s0 = rmin(ul - msqrt(gamma * pl * L.ir), ra[0] - ra[4]);
+      const R s2 = rmax(ur + msqrt(gamma * pr * Rt.ir), ra[0] + ra[4]);
+      const R Al = dl * (s0 - ul), Ar = dr * (s2 - ur);
+      const R s1 = (pr - pl + Al * ul - Ar * ur) / (Al - Ar);
+      const bool left = s1 >= R(0);
+      const R *q = left ? rl : rr;
+      const Prim &K = left ? L : Rt;
+      const R sK = left ? s0 : s2, AK = left ? Al : Ar;
+      const R wK = left ? rmin(s0, R(0)) : rmax(s2, R(0));
+      const R lm = AK / (sK - s1);
+      const R h[5] = {lm, lm * s1, lm * K.u[1], lm * K.u[2],
+                      lm * (q[4] * K.ir + (s1 - K.u[0]) * (s1 + K.p / AK))};
+      R fk[5];
+      flux1(q, K, fk);
+      for (int m = 0; m < 5; ++m) rf[m] = fk[m] + wK * (h[m] - q[m]);
     } else {
       R s[3], wave[15];
       [[maybe_unused]] R amdq[5], apdq[5];
