@@ -344,6 +344,22 @@ struct Euler {
   }
   template <int ND = 3>
   MXM_HD void numerical_flux(const R *ql, const R *qr, const R *n, R *f) const {
+    if constexpr (FLUX == 0) {
+      // Rusanov in the world frame: the rotated form below gives the same
+      // flux (normal velocity and pressure are frame invariant, the
+      // momentum terms rotate back), up to rounding, without the frame
+      const Prim L = prim(ql), Rt = prim(qr);
+      const R unl = L.u[0] * n[0] + L.u[1] * n[1] + L.u[2] * n[2];
+      const R unr = Rt.u[0] * n[0] + Rt.u[1] * n[1] + Rt.u[2] * n[2];
+      const R cl = msqrt(gamma * L.p * L.ir), cr = msqrt(gamma * Rt.p * Rt.ir);
+      const R cm = rmax(rabs(unl) + cl, rabs(unr) + cr);
+      const R mnl = ql[1] * n[0] + ql[2] * n[1] + ql[3] * n[2], mnr = qr[1] * n[0] + qr[2] * n[1] + qr[3] * n[2];
+      f[0] = R(0.5) * (mnl + mnr + cm * (L.r - Rt.r));
+      for (int d = 0; d < 3; ++d)
+        f[1 + d] = R(0.5) * (ql[1 + d] * unl + L.p * n[d] + qr[1 + d] * unr + Rt.p * n[d] + cm * (ql[1 + d] - qr[1 + d]));
+      f[4] = R(0.5) * ((ql[4] + L.p) * unl + (qr[4] + Rt.p) * unr + cm * (ql[4] - qr[4]));
+      return;
+    }
     R rl[5], rr[5], rf[5];
     if constexpr (ND == 2) {
       // n = (nx, ny, 0): the frame (n, (-ny, nx, 0), (0, 0, 1)).  The flux
