@@ -16,6 +16,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #define MXM_HD __host__ __device__ __forceinline__
 
 namespace mxm {
@@ -243,14 +245,16 @@ struct Euler {
       rf[2] = R(0.5) * (rl[2] * ul + rr[2] * ur + cm * (rl[2] - rr[2]));
       rf[3] = R(0.5) * (rl[3] * ul + rr[3] * ur + cm * (rl[3] - rr[3]));
       rf[4] = R(0.5) * ((rl[4] + pl) * ul + (rr[4] + pr) * ur + cm * (rl[4] - rr[4]));
-    } else if constexpr (FLUX == 3 && !efix) {
+    } else if constexpr (FLUX == 3 && !efix && std::is_same<R, double>::value) {
       // HLLEC without the entropy fix in its upwind-side form:
       // F = F_K + s_K^- (U*_K - U_K) on the side K of the contact (s_L^- =
       // min(s_L, 0) on the left, s_R^+ = max(s_R, 0) on the right).  The
       // star states satisfy the jump conditions across each of the three
       // waves, so this equals the wave sum of src/euler.mojo:415-520
       // (1/2 (F_L + F_R) - 1/2 sum_w |s_w| W_w) up to rounding, with one
-      // star state and one physical flux instead of two.
+      // star state and one physical flux instead of two.  FP64 only: in
+      // FP32 its rounding is larger than the wave sum's (the 128^2 P=3
+      // vortex at T=2 drifts 1.06e-5 from FP64 against 9.67e-6).
       R ra[5];
       roe_avg(rl, rr, L, Rt, ra);
       const R s0 = rmin(ul - msqrt(gamma * pl * L.ir), ra[0] - ra[4]);
