@@ -12,7 +12,8 @@ from .mesh import LocalMesh, LocalMesh2D
 from .reference import ReferenceElement, ReferenceElement2D, num_tet_nodes, num_tri_nodes
 from .solver import PREC_FP32_MFMA, PREC_FP64, Solver
 from .ssprk3 import SSPRK3StagePlan, ssprk3_stage_plans
-from .time_integrator import TimeLoopResult, apriori_dt, cfl_dt, fixed_steps, run_ssprk3_loop
+from .time_integrator import (TimeLoopResult, apriori_dt, cfl_dt, fixed_steps, run_ssprk3_loop,
+                              run_ssprk3_loop_with_diagnostics)
 
 __all__ = [
     "batch", "diagnostics", "mxmq", "serve", "DiagComponents", "Diagnostics", "DiagnosticsWriter",
@@ -20,4 +21,5 @@ __all__ = [
     "BoundaryConditions", "BoundaryConditions2D", "MemoryReport", "ThroughputReport", "LocalMesh", "LocalMesh2D",
     "ReferenceElement", "ReferenceElement2D", "num_tet_nodes", "num_tri_nodes", "Solver", "SSPRK3StagePlan",
     "ssprk3_stage_plans", "TimeLoopResult", "apriori_dt", "fixed_steps", "run_ssprk3_loop",
+    "run_ssprk3_loop_with_diagnostics", "cfl_dt", "vtu",
 ]

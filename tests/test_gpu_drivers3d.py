@@ -130,3 +130,33 @@ def test_frame_writer_writes_device_state(tmp_path):
     assert np.array_equal(r["fields"]["Bz"], q[..., 5].astype(np.float32).ravel())
     assert np.abs(r["fields"]["Bz"]).max() > 0.0
     assert open(pvd).read().count("<DataSet") == 2
+
+
+def test_time_loop_with_diagnostics(tmp_path):
+    """run_ssprk3_loop_with_diagnostics on a running Solver: a frame and a
+    CSV row at t = 0 and at each frame boundary, the last frame is the
+    final state, and the state matches the same steps taken in one
+    step_ssprk3 call (the clamped steps differ only by round-off)."""
+    from mojoxm_amd import vtu
+    p = pr.maxwell_cavity_3d(NY=8, t_final=0.2)
+    mesh = M.LocalMesh(*p.n, *p.L, P=p.P)
+    q0 = p.ic(mesh.elem_node_xyz)
+    comps = M.DiagComponents().squared("Ex2", 0).maxabs("Bz_max", 5)
+    with M.Solver(p.n, p.L, p.physics, P=p.P, bcs=p.bcs) as s:
+        s.set_state(q0)
+        fw = vtu.FrameWriter(s, mesh.elem_node_xyz, str(tmp_path), components={"Ex": 0})
+        dw = M.DiagnosticsWriter(s, str(tmp_path / "diag.csv"), comps)
+        r = M.run_ssprk3_loop_with_diagnostics(s, fw, dw, p.dt, p.t_final, num_frames=2)
+        q = s.get_state()
+        s.set_state(q0)
+        s.step_ssprk3(p.dt, p.nsteps)
+        q_ref = s.get_state()
+    print(p.nsteps, r, flush=True)
+    assert r.total_steps in (p.nsteps, p.nsteps + 1, p.nsteps + 2)
+    assert fw.times == pytest.approx([0.0, 0.1, 0.2])
+    rows = open(tmp_path / "diag.csv").read().splitlines()
+    assert rows[0] == "time,Ex2,Bz_max" and len(rows) == 4
+    assert float(rows[1].split(",")[2]) == 0.0 and float(rows[3].split(",")[2]) > 0.0
+    last = vtu.read_vtu(str(tmp_path / fw.paths[-1]))
+    assert np.array_equal(last["fields"]["Ex"], q[..., 0].astype(np.float32).ravel())
+    assert rel_l2(q, q_ref) < 1e-12
