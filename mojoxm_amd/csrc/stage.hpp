@@ -250,12 +250,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   // 16 KB (2D, 3D P <= 3), else read from global memory (L2) in the loop
   constexpr int NA = kMFMA ? MT * 16 * KPAD : 0;
   constexpr bool kStageA = kMFMA && NA * 4 <= 16384;
+  // LDS row stride of the staged operator: 4 mod 8 words, so the 16 rows
+  // x 4 columns an A operand read touches fall in 64 distinct banks (a
+  // stride of 32, as 2D P = 3 has, put all 16 rows in one bank group)
+  constexpr int KAS = KPAD % 8 == 4 ? KPAD : KPAD + 4;
   // sW = sB: first the q_in tile as [el][j][c] (TILE contiguous values);
   // after the lift product the contravariant nodal fluxes as [k][j][el][c]
   // (row kj at kj*NCOLS); after the volume product the result as
   // [el][i][c].  sS = sB + TILE: face fluxes [f*NFP + r][el][c].
   __shared__ R sB[NB];
-  __shared__ float sA[kStageA ? NA : 1];
+  __shared__ float sA[kStageA ? MT * 16 * KAS : 1];
   __shared__ double sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
   R *const sW = sB;
@@ -328,8 +332,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   if (tid < NF * NFP) sMap[tid] = fm;
   if constexpr (kStageA) {
 #pragma unroll
-    for (int k = 0; k < APT; ++k)
-      if (k * S::BLOCK + tid < NA) sA[k * S::BLOCK + tid] = ta[k];
+    for (int k = 0; k < APT; ++k) {
+      const int t = k * S::BLOCK + tid, row = t / KPAD;
+      if (t < NA) sA[row * KAS + t - row * KPAD] = ta[k];
+    }
   }
   if constexpr (kMFMA)
     for (int t = TILE + NFF * NCOLS + tid; t < NB; t += S::BLOCK) sB[t] = R(0);
@@ -446,11 +452,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     float a[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
-      const int ai = (m * 16 + lr) * KPAD + k0 + lg;
       if constexpr (kStageA)
-        a[m] = sA[ai];
+        a[m] = sA[(m * 16 + lr) * KAS + k0 + lg];
       else
-        a[m] = A.Aop[ai];
+        a[m] = A.Aop[(m * 16 + lr) * KPAD + k0 + lg];
     }
 #pragma unroll
     for (int w = 0; w < NTW; ++w) {
