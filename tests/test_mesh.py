@@ -2,6 +2,8 @@
 src/local_mesh.mojo / src/local_mesh_2d.mojo (after
 test/local_mesh_2d_test.mojo:36-106 and their 3D analogues), and bit-exact
 agreement of the library's independently derived builder with it."""
+import os
+
 import numpy as np
 import pytest
 
@@ -165,3 +167,28 @@ def test_library_mesh2d_one_cell_per_period(oracle, n, bc):
         om = oracle.Mesh(2, P, n, L, bc)
         for k in INT_TABLES:
             assert (getattr(pm, k) == getattr(om, k)).all(), (P, k)
+
+
+# ------------------------------------------- fixtures from the literal tables --
+GOLDEN_MESH = os.path.join(os.path.dirname(__file__), "golden", "kuhn_mesh_golden.npz")
+
+
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_kuhn_mesh_matches_literal_table_fixtures(oracle, P):
+    """The oracle's and the library's periodic 2x2x2 meshes equal the
+    connectivity that tests/golden/make_mesh_golden.py builds geometrically
+    from the reference's literal Kuhn tables (kuhn_vertex, tet_face_info,
+    face_type_corner, corner_d*; src/local_mesh.mojo:64-292), read from its
+    source as text."""
+    g = np.load(GOLDEN_MESH)
+    want = {k: g[f"{k}_p{P}"] for k in ("elem_faces", "elem_face_side", "face_elem", "face_elem_node",
+                                        "elem_canon_to_ref", "face_normal", "face_area")}
+    meshes = {"oracle": oracle.Mesh(3, P, (2, 2, 2), (1.0, 1.0, 1.0)), "library": M.LocalMesh(2, 2, 2, P=P)}
+    for who, m in meshes.items():
+        for k, v in want.items():
+            got = getattr(m, k)
+            if v.dtype.kind == "i":
+                assert np.array_equal(got, v), (who, k)
+            else:
+                assert np.abs(got - v).max() < 1e-15, (who, k)
+        assert np.array_equal(m.face_bc_type, np.zeros(m.Nf, np.int32)), who
