@@ -59,6 +59,33 @@ struct BInst {
 
 using GroupKey = std::tuple<int, int, int, int, int, long long>;  // dim, P, phys, variant, limiter, eps bits
 
+// The device arena of a kernel group: state trio, concatenated per-slot and
+// per-job tables, per-instance scalars.  A repack builds a whole new arena
+// before it replaces the old one, so a failed allocation or copy leaves the
+// group as it was.
+struct Arena {
+  DevBuf<double> qd[3];
+  DevBuf<float> qf[3];
+  DevBuf<double> mean, invJ, inv_vol, jgeo, iprm, idt, ipost;
+  DevBuf<int4> conn, jobs;
+  DevBuf<unsigned short> ipr;
+  DevBuf<int> tile_count, tile_inst, iactive, isteps, insteps;
+  void swap(Arena &o) noexcept {
+    for (int k = 0; k < 3; ++k) qd[k].swap(o.qd[k]), qf[k].swap(o.qf[k]);
+    mean.swap(o.mean), invJ.swap(o.invJ), inv_vol.swap(o.inv_vol), jgeo.swap(o.jgeo), iprm.swap(o.iprm);
+    idt.swap(o.idt), ipost.swap(o.ipost), conn.swap(o.conn), jobs.swap(o.jobs), ipr.swap(o.ipr);
+    tile_count.swap(o.tile_count), tile_inst.swap(o.tile_inst), iactive.swap(o.iactive), isteps.swap(o.isteps);
+    insteps.swap(o.insteps);
+  }
+  size_t bytes() const {
+    size_t b = 0;
+    for (int k = 0; k < 3; ++k) b += qd[k].bytes() + qf[k].bytes();
+    return b + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + iprm.bytes() + idt.bytes() +
+           ipost.bytes() + conn.bytes() + jobs.bytes() + ipr.bytes() + tile_count.bytes() + tile_inst.bytes() +
+           iactive.bytes() + isteps.bytes() + insteps.bytes();
+  }
+};
+
 struct BGroup {
   int dim = 0, P = 0, phys = 0, variant = 0, NC = 0, NP = 0, NFP = 0, NF = 0, epb = 0, prec = 0;
   int cfg = 0;  // launch configuration (StageCfg), autotuned per packed size
@@ -83,33 +110,23 @@ struct BGroup {
     for (size_t k = 0; k < rem.size() && rem[k] > s; ++k) t = end_tile[k];
     return t;
   }
-  DevBuf<double> qd[3];
-  DevBuf<float> qf[3];
-  DevBuf<double> mean, invJ, inv_vol, jgeo, Dt, Lt, w, iprm, idt, ipost;
+  Arena ar;
+  DevBuf<double> Dt, Lt, w;
   DevBuf<double> Dn, dpart, dout;  // diagnostics
   DevBuf<float> Aop;
-  DevBuf<int4> conn, jobs;
-  DevBuf<unsigned short> ipr;
-  DevBuf<int> fmap, tile_count, tile_inst, iactive, isteps, insteps;
+  DevBuf<int> fmap;
   // replayed HIP graphs of kGraphSteps steps ([0]) and of one step ([1])
   static constexpr int kGraphSteps = 8;
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  StepGraph sg[2];
   GroupKey key{};
 
   ~BGroup() { drop_graph(); }
   void drop_graph() {
-    for (auto &e : exec) {
-      if (e) (void)hipGraphExecDestroy(e);
-      e = nullptr;
-    }
+    for (auto &e : sg) e.reset();
   }
   size_t bytes() const {
-    size_t b = 0;
-    for (int k = 0; k < 3; ++k) b += qd[k].bytes() + qf[k].bytes();
-    return b + Dn.bytes() + dpart.bytes() + dout.bytes() + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + Dt.bytes() + Lt.bytes() + w.bytes() +
-           iprm.bytes() + idt.bytes() + ipost.bytes() + Aop.bytes() + conn.bytes() + jobs.bytes() + ipr.bytes() +
-           fmap.bytes() + tile_count.bytes() + tile_inst.bytes() + iactive.bytes() + isteps.bytes() +
-           insteps.bytes();
+    return ar.bytes() + Dn.bytes() + dpart.bytes() + dout.bytes() + Dt.bytes() + Lt.bytes() + w.bytes() +
+           Aop.bytes() + fmap.bytes();
   }
 };
 
@@ -234,66 +251,77 @@ struct Batch {
 
   // host mirrors of the device step counters of the packed arena
   void pull_counters(BGroup &g) {
-    if (g.packed.empty() || !g.isteps.p) return;
+    if (g.packed.empty() || !g.ar.isteps.p) return;
     std::vector<int> st(g.packed.size());
-    MXM_HIP(hipMemcpy(st.data(), g.isteps.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+    MXM_HIP(hipMemcpy(st.data(), g.ar.isteps.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (size_t k = 0; k < g.packed.size(); ++k) {
       auto it = insts.find(g.packed[k]);
       if (it != insts.end() && it->second->placed) it->second->steps_done = st[k];
     }
   }
 
+  // The new arena's state trio: placed members' tiles copied from the old
+  // arena (their old base tiles are still in BInst::base_tile), new members
+  // uploaded from q0.
   template <class R>
-  void place_states(BGroup &g, DevBuf<R> (&q)[3], const std::vector<int> &old_base) {
+  void place_states(const BGroup &g, DevBuf<R> (&q)[3], DevBuf<R> (&nq)[3], const std::vector<int> &order,
+                    const std::vector<int> &new_base) {
     const size_t tile_vals = (size_t)g.epb * g.NP * g.NC;
-    DevBuf<R> nq[3];
     for (auto &b : nq) {
       b.alloc(std::max<size_t>(1, (size_t)g.ntiles * tile_vals));
       MXM_HIP(hipMemset(b.p, 0, b.bytes()));
     }
-    for (size_t k = 0; k < g.members.size(); ++k) {
-      BInst &in = inst(g.members[k]);
-      R *dst = nq[0].p + (size_t)in.base_tile * tile_vals;
+    for (size_t k = 0; k < order.size(); ++k) {
+      const BInst &in = inst(order[k]);
+      R *dst = nq[0].p + (size_t)new_base[k] * tile_vals;
       const size_t nv = (size_t)in.t.til.ntiles * tile_vals;
       if (in.placed) {
-        MXM_HIP(hipMemcpy(dst, q[0].p + (size_t)old_base[k] * tile_vals, nv * sizeof(R), hipMemcpyDeviceToDevice));
+        MXM_HIP(hipMemcpy(dst, q[0].p + (size_t)in.base_tile * tile_vals, nv * sizeof(R), hipMemcpyDeviceToDevice));
       } else {
         const size_t node = (size_t)g.NP * g.NC;
         std::vector<R> d(nv, R(0));
         for (int e = 0; e < in.K; ++e)
           for (size_t c = 0; c < node; ++c) d[(size_t)in.t.til.dev_of[e] * node + c] = R(in.q0[(size_t)e * node + c]);
         MXM_HIP(hipMemcpy(dst, d.data(), nv * sizeof(R), hipMemcpyHostToDevice));
-        in.placed = true;
-        in.q0.clear();
-        in.q0.shrink_to_fit();
       }
     }
-    for (int k = 0; k < 3; ++k) q[k].swap(nq[k]);
   }
 
+  // Rebuilds the group's arena for its current member list.  Everything is
+  // built in a new arena first; only when every allocation and copy has
+  // succeeded are the arena, the member order, the base tiles and the
+  // placed flags replaced, so an error leaves the group as it was (still
+  // dirty, to be retried).
   void repack(BGroup &g) {
     MXM_HIP(hipStreamSynchronize(stream));
     pull_counters(g);
     g.drop_graph();
-    std::stable_sort(g.members.begin(), g.members.end(), [&](int a, int b) {
+    std::vector<int> order = g.members;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
       const BInst &x = inst(a), &y = inst(b);
       return x.nsteps - x.steps_done > y.nsteps - y.steps_done;
     });
-    std::vector<int> old_base(g.members.size());
+    std::vector<int> new_base(order.size());
     int tiles = 0, maxj = 1;
-    for (size_t k = 0; k < g.members.size(); ++k) {
-      BInst &in = inst(g.members[k]);
-      old_base[k] = in.base_tile;
-      in.base_tile = tiles;
+    for (size_t k = 0; k < order.size(); ++k) {
+      const BInst &in = inst(order[k]);
+      new_base[k] = tiles;
       tiles += in.t.til.ntiles;
       maxj = std::max(maxj, in.t.fj.maxj);
     }
-    g.ntiles = tiles;
-    g.maxj = maxj;
-    if (g.prec == MXM_PREC_FP64)
-      place_states(g, g.qd, old_base);
-    else
-      place_states(g, g.qf, old_base);
+    const int old_ntiles = g.ntiles;
+    g.ntiles = tiles;  // place_states sizes the new trio from it
+    Arena na;
+    try {
+      if (g.prec == MXM_PREC_FP64)
+        place_states(g, g.ar.qd, na.qd, order, new_base);
+      else
+        place_states(g, g.ar.qf, na.qf, order, new_base);
+    } catch (...) {
+      g.ntiles = old_ntiles;
+      throw;
+    }
+    g.ntiles = old_ntiles;
     // concatenated tables, slots offset by each instance's base
     const int epb = g.epb, dd = g.dim * g.dim, NF = g.NF, NFP = g.NFP;
     const size_t slots = (size_t)tiles * epb;
@@ -301,13 +329,13 @@ struct Batch {
     std::vector<int4> cn(slots * NF), jb((size_t)tiles * maxj);
     std::vector<unsigned short> pr((size_t)tiles * maxj * NFP, 0);
     std::vector<int> tc(tiles, 0), ti(tiles, 0);
-    const size_t ni = g.members.size();
+    const size_t ni = order.size();
     std::vector<double> iprm(ni * MXM_PRM_MAX), idt(ni), ipost(ni);
     std::vector<int> iact(ni), ist(ni), inst_n(ni);
     for (size_t k = 0; k < ni; ++k) {
-      BInst &in = inst(g.members[k]);
+      const BInst &in = inst(order[k]);
       const ElemTables &t = in.t;
-      const int bt = in.base_tile, bs = bt * epb, lt = t.til.ntiles, lmj = t.fj.maxj;
+      const int bt = new_base[k], bs = bt * epb, lt = t.til.ntiles, lmj = t.fj.maxj;
       for (int s = 0; s < lt * epb; ++s) {
         for (int q = 0; q < dd; ++q) ij[(size_t)(bs + s) * dd + q] = t.invJ[(size_t)s * dd + q];
         iv[bs + s] = t.inv_vol[s];
@@ -340,27 +368,41 @@ struct Batch {
       inst_n[k] = in.nsteps;
       iact[k] = in.steps_done < in.nsteps ? 1 : 0;
     }
-    g.invJ.upload(ij.data(), ij.size());
-    g.inv_vol.upload(iv.data(), iv.size());
-    g.conn.upload(cn.data(), cn.size());
-    g.jobs.upload(jb.data(), jb.size());
-    g.jgeo.upload(geo.data(), geo.size());
-    g.ipr.upload(pr.data(), pr.size());
-    g.tile_count.upload(tc.data(), tc.size());
-    g.tile_inst.upload(ti.data(), ti.size());
-    g.iprm.upload(iprm.data(), iprm.size());
-    g.idt.upload(idt.data(), idt.size());
-    g.ipost.upload(ipost.data(), ipost.size());
-    g.iactive.upload(iact.data(), iact.size());
-    g.isteps.upload(ist.data(), ist.size());
-    g.insteps.upload(inst_n.data(), inst_n.size());
-    if (g.limiter) g.mean.alloc(slots * g.NC);
-    g.packed = g.members;
+    na.invJ.upload(ij.data(), ij.size());
+    na.inv_vol.upload(iv.data(), iv.size());
+    na.conn.upload(cn.data(), cn.size());
+    na.jobs.upload(jb.data(), jb.size());
+    na.jgeo.upload(geo.data(), geo.size());
+    na.ipr.upload(pr.data(), pr.size());
+    na.tile_count.upload(tc.data(), tc.size());
+    na.tile_inst.upload(ti.data(), ti.size());
+    na.iprm.upload(iprm.data(), iprm.size());
+    na.idt.upload(idt.data(), idt.size());
+    na.ipost.upload(ipost.data(), ipost.size());
+    na.iactive.upload(iact.data(), iact.size());
+    na.isteps.upload(ist.data(), ist.size());
+    na.insteps.upload(inst_n.data(), inst_n.size());
+    if (g.limiter) na.mean.alloc(slots * g.NC);
+    // ---- commit (no more allocations or copies that can fail) ----
+    g.ar.swap(na);
+    g.ntiles = tiles;
+    g.maxj = maxj;
+    g.members = order;
+    g.packed = order;
+    for (size_t k = 0; k < ni; ++k) {
+      BInst &in = inst(order[k]);
+      in.base_tile = new_base[k];
+      if (!in.placed) {
+        in.placed = true;
+        in.q0.clear();
+        in.q0.shrink_to_fit();
+      }
+    }
     g.dirty = false;
     g.rem.resize(ni);
     g.end_tile.resize(ni);
     for (size_t k = 0; k < ni; ++k) {
-      const BInst &in = inst(g.members[k]);
+      const BInst &in = inst(order[k]);
       g.rem[k] = std::max(0, in.nsteps - in.steps_done);
       g.end_tile[k] = in.base_tile + in.t.til.ntiles;
     }
@@ -392,12 +434,13 @@ struct Batch {
       MXM_HIP(hipStreamSynchronize(stream));
       int c;
       if (g.prec == MXM_PREC_FP64) {
-        const StageArgs<double> a = stage_args(g, g.qd, 0);
+        const StageArgs<double> a = stage_args(g, g.ar.qd, 0);
         c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, stream, 10, nullptr);
       } else {
-        const StageArgs<float> a = stage_args(g, g.qf, 0);
+        const StageArgs<float> a = stage_args(g, g.ar.qf, 0);
         c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, stream, 10, nullptr);
       }
+      MXM_HIP(hipStreamSynchronize(stream));
       it = tuned.emplace(key, c).first;
     }
     g.cfg = it->second;
@@ -405,20 +448,21 @@ struct Batch {
   }
 
   template <class R>
-  StageArgs<R> stage_args(const BGroup &g, DevBuf<R> (&q)[3], int s) const {
+  StageArgs<R> stage_args(const BGroup &g, const DevBuf<R> (&q)[3], int s) const {
     StageArgs<R> a{};
     const StagePlan pl = stage_plan(s);
+    const Arena &ar = g.ar;
     a.q_in = q[pl.in].p, a.q_a = q[pl.qa].p, a.q_out = q[pl.out].p;
     a.a = pl.a, a.b = pl.b, a.c = pl.c;
-    a.tile_inst = g.tile_inst.p, a.inst_prm = g.iprm.p, a.inst_dt = g.idt.p, a.inst_post = g.ipost.p;
-    a.inst_active = g.iactive.p;
+    a.tile_inst = ar.tile_inst.p, a.inst_prm = ar.iprm.p, a.inst_dt = ar.idt.p, a.inst_post = ar.ipost.p;
+    a.inst_active = ar.iactive.p;
     a.dt = 0.0, a.post_scale = 1.0;
     // GLM decay per instance (factor 1 where alpha_d = 0)
     a.post_comp = s == 2 && g.phys == MXM_PHYS_MHDGLM2D ? 6 : -1;
-    a.tile_count = g.tile_count.p;
-    a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p;
+    a.tile_count = ar.tile_count.p;
+    a.invJ = ar.invJ.p, a.inv_vol = ar.inv_vol.p;
     a.Dt = g.Dt.p, a.Lt = g.Lt.p, a.Aop = g.Aop.p, a.fmap = g.fmap.p;
-    a.jobs = g.jobs.p, a.jgeo = g.jgeo.p, a.ipr = g.ipr.p, a.maxj = g.maxj, a.ntiles = g.grid_tiles;
+    a.jobs = ar.jobs.p, a.jgeo = ar.jgeo.p, a.ipr = ar.ipr.p, a.maxj = g.maxj, a.ntiles = g.grid_tiles;
     return a;
   }
 
@@ -429,24 +473,24 @@ struct Batch {
       const StageArgs<R> a = stage_args(g, q, s);
       MXM_HIP(g.launch(&a, stream));
       if (g.limiter)
-        MXM_HIP(launch_limiter<R>(a.q_out, g.mean.p, g.w.p, g.conn.p, g.grid_tiles * g.epb, g.NP, g.NC, g.NF, g.dim,
-                                  g.eps, stream, g.tile_inst.p, g.iactive.p, g.epb));
+        MXM_HIP(launch_limiter<R>(a.q_out, g.ar.mean.p, g.w.p, g.ar.conn.p, g.grid_tiles * g.epb, g.NP, g.NC, g.NF,
+                                  g.dim, g.eps, stream, g.ar.tile_inst.p, g.ar.iactive.p, g.epb));
     }
-    hipLaunchKernelGGL(bump_steps_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, g.isteps.p, g.insteps.p,
-                       g.iactive.p, ni);
+    hipLaunchKernelGGL(bump_steps_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, g.ar.isteps.p,
+                       g.ar.insteps.p, g.ar.iactive.p, ni);
     MXM_HIP(hipGetLastError());
   }
 
   void launch_step(BGroup &g) {
     if (g.prec == MXM_PREC_FP64)
-      launch_step(g, g.qd);
+      launch_step(g, g.ar.qd);
     else
-      launch_step(g, g.qf);
+      launch_step(g, g.ar.qf);
   }
 
-  // kGraphSteps SSPRK3 steps, and one step, of the group as HIP graphs
-  // (kernel nodes only)
-  hipGraphExec_t capture_steps(BGroup &g, int n) {
+  // n SSPRK3 steps of the group as one HIP graph (kernel nodes only); the
+  // graph is kept beside its executable
+  void capture_steps(BGroup &g, int n, StepGraph &out) {
     MXM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
       for (int k = 0; k < n; ++k) launch_step(g);
@@ -456,17 +500,12 @@ struct Batch {
       if (tmp) (void)hipGraphDestroy(tmp);
       throw;
     }
-    hipGraph_t gr = nullptr;
-    MXM_HIP(hipStreamEndCapture(stream, &gr));
-    hipGraphExec_t e = nullptr;
-    const hipError_t rc = hipGraphInstantiate(&e, gr, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(gr);
-    MXM_HIP(rc);
-    return e;
+    MXM_HIP(hipStreamEndCapture(stream, &out.graph));
+    MXM_HIP(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
   }
   void capture(BGroup &g) {
-    g.exec[0] = capture_steps(g, BGroup::kGraphSteps);
-    g.exec[1] = capture_steps(g, 1);
+    capture_steps(g, BGroup::kGraphSteps, g.sg[0]);
+    capture_steps(g, 1, g.sg[1]);
   }
 
   void step(int n) {
@@ -482,7 +521,7 @@ struct Batch {
         g.launched += k;
         if (need == 0) continue;
         if (need <= g.grid_tiles - g.grid_tiles / 8) recapture(g, need);
-        MXM_HIP(hipGraphLaunch(g.exec[which], stream));
+        MXM_HIP(hipGraphLaunch(g.sg[which].exec, stream));
       }
     };
     for (; n >= BGroup::kGraphSteps; n -= BGroup::kGraphSteps) launch_all(0, BGroup::kGraphSteps);
@@ -515,9 +554,9 @@ struct Batch {
     if (g.dirty) repack(g);
     MXM_HIP(hipStreamSynchronize(stream));
     if (g.prec == MXM_PREC_FP64)
-      read_state(g, g.qd, in, h);
+      read_state(g, g.ar.qd, in, h);
     else
-      read_state(g, g.qf, in, h);
+      read_state(g, g.ar.qf, in, h);
   }
 
   // the largest signal speed over the nodes of one instance
@@ -529,17 +568,17 @@ struct Batch {
     if (g.dpart.n < (size_t)nt) g.dpart.alloc(nt);
     if (g.dout.n < 1) g.dout.alloc(1);
     auto fill = [&](auto &a) {
-      a.prm = g.iprm.p, a.tile_inst = g.tile_inst.p, a.tile_count = g.tile_count.p;
+      a.prm = g.ar.iprm.p, a.tile_inst = g.ar.tile_inst.p, a.tile_count = g.ar.tile_count.p;
       a.epb = g.epb, a.NP = g.NP, a.tile0 = in.base_tile, a.ntiles = nt, a.part = g.dpart.p, a.out = g.dout.p;
     };
     if (g.prec == MXM_PREC_FP64) {
       SpeedArgs<double> a{};
-      a.q = g.qd[0].p;
+      a.q = g.ar.qd[0].p;
       fill(a);
       MXM_HIP(launch_max_speed(g.phys, true, &a, stream));
     } else {
       SpeedArgs<float> a{};
-      a.q = g.qf[0].p;
+      a.q = g.ar.qf[0].p;
       fill(a);
       MXM_HIP(launch_max_speed(g.phys, false, &a, stream));
     }
@@ -559,18 +598,18 @@ struct Batch {
     if (g.dpart.n < (size_t)nt * per) g.dpart.alloc((size_t)nt * per);
     if (g.dout.n < (size_t)(3 * NC + 2)) g.dout.alloc(3 * NC + 2);
     auto fill = [&](auto &a) {
-      a.invJ = g.invJ.p, a.inv_vol = g.inv_vol.p, a.w = g.w.p, a.Dn = g.Dn.p, a.tile_count = g.tile_count.p;
+      a.invJ = g.ar.invJ.p, a.inv_vol = g.ar.inv_vol.p, a.w = g.w.p, a.Dn = g.Dn.p, a.tile_count = g.ar.tile_count.p;
       a.epb = g.epb, a.tile0 = in.base_tile, a.ntiles = nt, a.NC = NC, a.nvel = diag_nvel(g.phys);
       a.part = g.dpart.p, a.pmax = g.dpart.p + (size_t)nt * kDiagSums * 2, a.out = g.dout.p;
     };
     if (g.prec == MXM_PREC_FP64) {
       DiagArgs<double> a{};
-      a.q = g.qd[0].p;
+      a.q = g.ar.qd[0].p;
       fill(a);
       MXM_HIP(launch_diagnostics(g.dim, g.P, true, &a, stream));
     } else {
       DiagArgs<float> a{};
-      a.q = g.qf[0].p;
+      a.q = g.ar.qf[0].p;
       fill(a);
       MXM_HIP(launch_diagnostics(g.dim, g.P, false, &a, stream));
     }

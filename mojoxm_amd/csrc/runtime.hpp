@@ -86,6 +86,20 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// A captured step sequence: the graph and its executable live and die
+// together (a profiler or the runtime may still refer to the graph's nodes
+// while the executable is launched).
+struct StepGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
 inline int phys_nc(int phys) {
   static const int nc[11] = {1, 5, 4, 3, 7, 6, 1, 3, 6, 9, 17};
   return (phys >= 0 && phys < 11) ? nc[phys] : -1;

@@ -56,7 +56,7 @@ struct Solver {
   // any of them changed.  A replayed graph costs about 1 us less per kernel
   // than stream launches of the same kernels.
   static constexpr int kGraphSteps = 8;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  StepGraph gexec[2];
   double g_dt = 0.0, g_eps = 0.0;
   bool g_lim = false;
   int g_cfg = -1;
@@ -67,13 +67,11 @@ struct Solver {
   }
 
   void drop_graphs() {
-    for (auto &e : gexec) {
-      if (e) (void)hipGraphExecDestroy(e);
-      e = nullptr;
-    }
+    for (auto &e : gexec) e.reset();
   }
 
-  hipGraphExec_t capture_steps(double dt, int n) {
+  // n SSPRK3 steps as one HIP graph, kept beside its executable
+  void capture_steps(double dt, int n, StepGraph &out) {
     MXM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     try {
       for (int k = 0; k < n; ++k)
@@ -84,13 +82,8 @@ struct Solver {
       if (tmp) (void)hipGraphDestroy(tmp);
       throw;
     }
-    hipGraph_t g = nullptr;
-    MXM_HIP(hipStreamEndCapture(stream, &g));
-    hipGraphExec_t e = nullptr;
-    const hipError_t rc = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MXM_HIP(rc);
-    return e;
+    MXM_HIP(hipStreamEndCapture(stream, &out.graph));
+    MXM_HIP(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
   }
 
   void build(const HostMesh &m, const RefElem &re) {
@@ -241,10 +234,10 @@ struct Solver {
       drop_graphs();
       g_dt = dt, g_lim = limiter, g_eps = eps, g_cfg = cfg;
     }
-    if (n >= kGraphSteps && !gexec[0]) gexec[0] = capture_steps(dt, kGraphSteps);
-    if (n % kGraphSteps && !gexec[1]) gexec[1] = capture_steps(dt, 1);
-    for (; n >= kGraphSteps; n -= kGraphSteps) MXM_HIP(hipGraphLaunch(gexec[0], stream));
-    for (; n > 0; --n) MXM_HIP(hipGraphLaunch(gexec[1], stream));
+    if (n >= kGraphSteps && !gexec[0].exec) capture_steps(dt, kGraphSteps, gexec[0]);
+    if (n % kGraphSteps && !gexec[1].exec) capture_steps(dt, 1, gexec[1]);
+    for (; n >= kGraphSteps; n -= kGraphSteps) MXM_HIP(hipGraphLaunch(gexec[0].exec, stream));
+    for (; n > 0; --n) MXM_HIP(hipGraphLaunch(gexec[1].exec, stream));
   }
 
   long long dof() const { return (long long)K * NP * NC; }
@@ -627,7 +620,12 @@ int mxm_dg_time_stage_kernel(void *h, double dt, int nsteps, double *avg_us) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (!avg_us || nsteps <= 0) throw ArgError("bad timing arguments");
-    const bool lim = s->limiter;
+    // the limiter is off for the timed steps and restored on every exit
+    struct Restore {
+      Solver *s;
+      bool lim;
+      ~Restore() { s->limiter = lim; }
+    } restore{s, s->limiter};
     s->limiter = false;
     // two events on the solver's stream around nsteps SSPRK3 steps, launched
     // as the stepping loop launches them (replayed graphs): the average
@@ -643,7 +641,6 @@ int mxm_dg_time_stage_kernel(void *h, double dt, int nsteps, double *avg_us) {
     float ms = 0.f;
     MXM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     for (auto &e : ev) (void)hipEventDestroy(e);
-    s->limiter = lim;
     *avg_us = (double)ms * 1e3 / (3.0 * nsteps);
     return MXM_OK;
   });
