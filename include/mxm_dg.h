@@ -128,6 +128,13 @@ typedef struct {
     double dof_per_second;          /* dof*steps/wall                   */
     double dof_stage_updates_per_second; /* 3 * dof_per_second          */
     double state_bandwidth_gbs;     /* lower-bound state GB/s           */
+    /* Device-side figures (no reference counterpart), from two HIP events
+     * on the solver's stream around the measured steps: */
+    double stage_seconds;     /* per stage (kernel + limiter when on, gaps included) */
+    double achieved_hbm_gbs;  /* state bytes per stage (state_bytes_per_step/3) / stage_seconds */
+    double hbm_fraction;      /* achieved_hbm_gbs / 8000 (MI355X HBM3E peak, GB/s) */
+    double mfma_tflops;       /* FP32-MFMA form: 2*dof*(ND*NP + NF*NFP) contraction flops per stage / stage_seconds; 0 in FP64 */
+    double mfma_utilization;  /* mfma_tflops / 157.3 (MI355X dense FP32 matrix peak); 0 in FP64 */
 } mxm_throughput;
 
 /* MemoryReport, src/memory_report.mojo:36-130 (six categories, bytes). */
@@ -188,9 +195,14 @@ int mxm_dg_download_component(void *solver, int c, double *out, long long n);
  * followed by the limiter when it is enabled (src/solver.mojo:1027-1036). */
 int mxm_dg_rk_stage(void *solver, int stage, double dt);
 /* nsteps x step_ssprk3, src/solver.mojo:1071-1092.  The steps are launched
- * as replayed HIP graphs of 8 steps and of 1 step, rebuilt when dt, the
- * limiter or the launch configuration changed since the last call. */
+ * as replayed HIP graphs, one of 32 steps and one of exactly the remainder
+ * nsteps mod 32, rebuilt when dt, the limiter or the launch configuration
+ * changed since the last call. */
 int mxm_dg_step_ssprk3(void *solver, double dt, int nsteps);
+/* Builds the graphs mxm_dg_step_ssprk3(solver, dt, nsteps) replays, without
+ * running a step (no reference counterpart: lets a caller keep graph
+ * construction out of a timed region). */
+int mxm_dg_prepare_steps(void *solver, double dt, int nsteps);
 /* enable_cell_limiter, src/solver.mojo:735-751 (eps: Venkatakrishnan). */
 int mxm_dg_enable_limiter(void *solver, int on, double eps);
 int mxm_dg_sync(void *solver);
@@ -207,7 +219,7 @@ int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
  * back-to-back launches (limiter off), from two HIP events on the solver's
  * own stream around nsteps SSPRK3 steps launched as mxm_dg_step_ssprk3
  * launches them (replayed HIP graphs), so the average includes the
- * launch-to-launch gaps; advances the state by nsteps + 9 steps.  No
+ * launch-to-launch gaps; advances the state by nsteps + 1 steps.  No
  * reference counterpart: it is the measurement hook that the roofline
  * figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
@@ -279,16 +291,27 @@ int mxm_batch_footprint_3d(const mxm_mesh3d_desc *desc, int P, int phys, int pre
 /* nsteps SSPRK3 steps of every unfinished instance (asynchronous). */
 int mxm_batch_step(void *batch, int nsteps);
 /* Synchronises; ids[0..*count) = instances that finished all their steps
- * since the last poll (at most cap). */
+ * (or failed, mxm_batch_failed) since the last poll (at most cap). */
 int mxm_batch_poll(void *batch, int *ids, int cap, int *count);
+/* mxm_batch_poll that waits only for the work of the mxm_batch_step calls
+ * before the most recent one, so the device stays busy with that one while
+ * the host handles the results (no reference counterpart: the serving
+ * loop's way of keeping work queued). */
+int mxm_batch_poll_lagged(void *batch, int *ids, int cap, int *count);
 int mxm_batch_status(void *batch, int id, int *steps_done);
+/* *failed = 1 when a stored value of the instance became non-finite: it
+ * was switched off at that step (the others go on) and poll reports it as
+ * done (no reference counterpart: the serving loop evicts such requests). */
+int mxm_batch_failed(void *batch, int id, int *failed);
 /* State of one instance, reference order [K*NP*NC], FP64. */
 int mxm_batch_get_state(void *batch, int id, double *q, long long n);
 /* mxm_dg_diagnostics of one instance. */
 int mxm_batch_diagnostics(void *batch, int id, double *out, int cap, int *n);
 /* mxm_dg_max_wave_speed of one instance. */
 int mxm_batch_max_wave_speed(void *batch, int id, double *speed);
-/* Drop an instance; its device memory is reclaimed at the next step. */
+/* Drop an instance.  A placed instance's tiles become a free range that
+ * later instances of its group can take without a compaction; its device
+ * memory is released when the group is next compacted. */
 int mxm_batch_remove(void *batch, int id);
 /* Device bytes held by the batch. */
 int mxm_batch_memory(void *batch, long long *bytes);

@@ -35,7 +35,9 @@ class Mesh2DDesc(C.Structure):
 class Throughput(C.Structure):
     _fields_ = [("steps", C.c_int), ("wall_seconds", C.c_double), ("dof", C.c_longlong),
                 ("state_bytes_per_step", C.c_longlong), ("dof_per_second", C.c_double),
-                ("dof_stage_updates_per_second", C.c_double), ("state_bandwidth_gbs", C.c_double)]
+                ("dof_stage_updates_per_second", C.c_double), ("state_bandwidth_gbs", C.c_double),
+                ("stage_seconds", C.c_double), ("achieved_hbm_gbs", C.c_double), ("hbm_fraction", C.c_double),
+                ("mfma_tflops", C.c_double), ("mfma_utilization", C.c_double)]
 
 
 class MemReport(C.Structure):
@@ -63,6 +65,7 @@ EXPORTS = {
     "mxm_dg_download_component": ([C.c_void_p, C.c_int, C.c_void_p, C.c_longlong], C.c_int),
     "mxm_dg_rk_stage": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
     "mxm_dg_step_ssprk3": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
+    "mxm_dg_prepare_steps": ([C.c_void_p, C.c_double, C.c_int], C.c_int),
     "mxm_dg_enable_limiter": ([C.c_void_p, C.c_int, C.c_double], C.c_int),
     "mxm_dg_sync": ([C.c_void_p], C.c_int),
     "mxm_dg_bench_step_loop": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.POINTER(Throughput)], C.c_int),
@@ -90,7 +93,9 @@ EXPORTS = {
                                 C.POINTER(C.c_longlong)], C.c_int),
     "mxm_batch_step": ([C.c_void_p, C.c_int], C.c_int),
     "mxm_batch_poll": ([C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mxm_batch_poll_lagged": ([C.c_void_p, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_status": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mxm_batch_failed": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_get_state": ([C.c_void_p, C.c_int, C.c_void_p, C.c_longlong], C.c_int),
     "mxm_batch_remove": ([C.c_void_p, C.c_int], C.c_int),
     "mxm_batch_memory": ([C.c_void_p, C.POINTER(C.c_longlong)], C.c_int),

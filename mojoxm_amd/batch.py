@@ -91,17 +91,27 @@ class Batch:
     def step(self, nsteps: int = 1) -> None:
         L.check(L.lib().mxm_batch_step(self._h, int(nsteps)), "mxm_batch_step")
 
-    def poll(self, cap: int = 4096) -> list:
-        """Synchronise; ids of instances that finished since the last poll."""
+    def poll(self, cap: int = 4096, lagged: bool = False) -> list:
+        """Ids of instances that finished (or failed) since the last poll.
+        Synchronises; with lagged=True it waits only for the work of the
+        step() calls before the most recent one, which keeps running."""
         ids = (C.c_int * cap)()
         cnt = C.c_int()
-        L.check(L.lib().mxm_batch_poll(self._h, ids, cap, C.byref(cnt)), "mxm_batch_poll")
+        fn = L.lib().mxm_batch_poll_lagged if lagged else L.lib().mxm_batch_poll
+        L.check(fn(self._h, ids, cap, C.byref(cnt)), "mxm_batch_poll")
         return list(ids[: cnt.value])
 
     def status(self, ident: int) -> int:
         s = C.c_int()
         L.check(L.lib().mxm_batch_status(self._h, ident, C.byref(s)), "mxm_batch_status")
         return s.value
+
+    def failed(self, ident: int) -> bool:
+        """True when a stored value of the instance became non-finite (it
+        was switched off at that step; poll reports it)."""
+        f = C.c_int()
+        L.check(L.lib().mxm_batch_failed(self._h, ident, C.byref(f)), "mxm_batch_failed")
+        return bool(f.value)
 
     def get_state(self, ident: int) -> np.ndarray:
         q = np.zeros(self._shape[ident])

@@ -4,26 +4,37 @@
 // An instance is one problem (mesh, order, physics parameters, time step,
 // step count, initial state).  Instances that use the same stage kernel
 // (dimension, P, physics, flux variant, limiter) form a group whose tiles
-// are concatenated in one device arena: the stage kernel's block reads its
-// tile's instance and from it the physics parameters and time step
+// share one device arena: the stage kernel's block reads its tile's
+// instance slot and from it the physics parameters and time step
 // (StageArgs::tile_inst / inst_*), so one launch advances every instance
-// of the group by one stage.  A device step counter per instance turns the
-// instance off (inst_active = 0) after its last step, so instances with
-// different step counts share the launches and finished ones cost a block
-// exit.  One SSPRK3 step of a group (three stages, limiter, counter bump)
-// is captured in a HIP graph and replayed.
+// of the group by one stage.  A device step counter per instance slot
+// turns the instance off (inst_active = 0) after its last step, or as soon
+// as one of its stored values is not finite (inst_fail, set by the stage
+// kernel), so instances with different step counts share the launches and
+// finished or failed ones cost a block exit.  One SSPRK3 step of a group
+// (three stages, limiter, counter bump) is a HIP graph, replayed.
 //
-// Adding or removing instances marks the group dirty; the next step
-// repacks it: a new arena in the new member order, the placed states
-// copied device to device, new states uploaded, the tables rebuilt.
-// Results do not depend on the mix: a tile never mixes instances and every
-// element's arithmetic only reads its own instance.
+// Every group has its own HIP stream: groups (different physics, orders,
+// limiter settings) run concurrently on the device.  A new instance is
+// placed into a free tile range of its group's arena -- a range a removed
+// instance left, or the arena's unused tail -- by asynchronous copies on
+// the group's stream, with no synchronisation.  Only when no range fits
+// (or the instance needs more face-job slots per tile than the arena has)
+// is the group compacted: its stream is drained, a new arena is built for
+// its placed and waiting instances (longest remaining run first, so the
+// grid shrinks as instances finish), and the old one is released.  The other
+// groups keep running meanwhile.
+//
+// Results do not depend on the mix: a tile never mixes instances, and
+// every element's arithmetic only reads its own instance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/mxm_dg.h"
@@ -35,13 +46,26 @@
 namespace mxm {
 namespace {
 
-__global__ void bump_steps_kernel(int *steps, const int *nsteps, int *active, int n) {
+// One SSPRK3 step done for every active slot; a slot whose state went
+// non-finite is switched off instead.
+__global__ void bump_steps_kernel(int *steps, const int *nsteps, int *active, const int *fail, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && active[i]) {
+    if (fail[i]) {
+      active[i] = 0;
+      return;
+    }
     const int s = steps[i] + 1;
     steps[i] = s;
     if (s >= nsteps[i]) active[i] = 0;
   }
+}
+
+// dst[0..n) = v (tile ranges handed back to the dead slot, switched-off
+// slots); a kernel rather than a memset so that it is ordinary stream work
+__global__ void fill_int_kernel(int *dst, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
 }
 
 struct BInst {
@@ -50,79 +74,152 @@ struct BInst {
   double eps = 0.1;
   double prm[MXM_PRM_MAX] = {0};
   double dt = 0.0;
-  int nsteps = 0, steps_done = 0;
-  bool placed = false, reported = false;
-  int base_tile = 0;
+  int nsteps = 0;
+  int steps_done = 0;      // host mirror of the device counter
+  int steps_enqueued = 0;  // steps launched for it so far (the host's own count)
+  bool failed = false, reported = false;
+  bool placed = false;
+  int base_tile = 0, slot = 0;
   std::vector<double> q0;  // initial state, reference order, until placed
   ElemTables t;
+  int ntiles() const { return t.til.ntiles; }
+  bool live() const { return !failed && steps_enqueued < nsteps; }
 };
 
 using GroupKey = std::tuple<int, int, int, int, int, long long>;  // dim, P, phys, variant, limiter, eps bits
 
-// The device arena of a kernel group: state trio, concatenated per-slot and
-// per-job tables, per-instance scalars.  A repack builds a whole new arena
-// before it replaces the old one, so a failed allocation or copy leaves the
-// group as it was.
+// The device arena of a kernel group, sized for FreeSpace::cap tiles and
+// cap_slots instance slots: state trio, per-slot and per-job tables,
+// per-instance scalars.
 struct Arena {
   DevBuf<double> qd[3];
   DevBuf<float> qf[3];
   DevBuf<double> mean, invJ, inv_vol, jgeo, iprm, idt, ipost;
   DevBuf<int4> conn, jobs;
   DevBuf<unsigned short> ipr;
-  DevBuf<int> tile_count, tile_inst, iactive, isteps, insteps;
+  DevBuf<int> tile_count, tile_inst, iactive, isteps, insteps, ifail;
   void swap(Arena &o) noexcept {
     for (int k = 0; k < 3; ++k) qd[k].swap(o.qd[k]), qf[k].swap(o.qf[k]);
     mean.swap(o.mean), invJ.swap(o.invJ), inv_vol.swap(o.inv_vol), jgeo.swap(o.jgeo), iprm.swap(o.iprm);
     idt.swap(o.idt), ipost.swap(o.ipost), conn.swap(o.conn), jobs.swap(o.jobs), ipr.swap(o.ipr);
     tile_count.swap(o.tile_count), tile_inst.swap(o.tile_inst), iactive.swap(o.iactive), isteps.swap(o.isteps);
-    insteps.swap(o.insteps);
+    insteps.swap(o.insteps), ifail.swap(o.ifail);
   }
   size_t bytes() const {
     size_t b = 0;
     for (int k = 0; k < 3; ++k) b += qd[k].bytes() + qf[k].bytes();
     return b + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + iprm.bytes() + idt.bytes() +
            ipost.bytes() + conn.bytes() + jobs.bytes() + ipr.bytes() + tile_count.bytes() + tile_inst.bytes() +
-           iactive.bytes() + isteps.bytes() + insteps.bytes();
+           iactive.bytes() + isteps.bytes() + insteps.bytes() + ifail.bytes();
+  }
+};
+
+// Pinned host memory for asynchronous uploads; busy until its event (on
+// the stream that reads it) completes.
+struct Staging {
+  char *p = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool busy = false;
+  ~Staging() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (p) (void)hipHostFree(p);
+  }
+};
+
+// Free tile ranges of an arena: holes inside [0, top) and the unused tail
+// [top, cap).
+struct FreeSpace {
+  int top = 0, cap = 0;
+  std::map<int, int> holes;  // start -> length
+  // a free range of nt tiles: a hole, else the tail; -1 if none
+  int find(int nt) const {
+    for (auto &h : holes)
+      if (h.second >= nt) return h.first;
+    return top + nt <= cap ? top : -1;
+  }
+  void take(int start, int nt) {
+    if (start == top) {
+      top += nt;
+      return;
+    }
+    auto it = holes.find(start);
+    const int len = it->second;
+    holes.erase(it);
+    if (len > nt) holes.emplace(start + nt, len - nt);
+  }
+  void give(int start, int nt) {
+    auto it = holes.emplace(start, nt).first;
+    auto nx = std::next(it);
+    if (nx != holes.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      holes.erase(nx);
+    }
+    if (it != holes.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        holes.erase(it);
+        it = pv;
+      }
+    }
+    if (it->first + it->second == top) {
+      top = it->first;
+      holes.erase(it);
+    }
   }
 };
 
 struct BGroup {
   int dim = 0, P = 0, phys = 0, variant = 0, NC = 0, NP = 0, NFP = 0, NF = 0, epb = 0, prec = 0;
-  int cfg = 0;  // launch configuration (StageCfg), autotuned per packed size
+  int cfg = 0;  // launch configuration (StageCfg), autotuned per grid size
   bool limiter = false;
   double eps = 0.1;
   StageLaunch launch = nullptr;
-  std::vector<int> members;  // instance ids in arena order
-  std::vector<int> packed;   // the member list the device arena holds
-  bool dirty = true;
-  int ntiles = 0, maxj = 0;
-  // The arena orders the members by their remaining steps at the repack,
-  // longest first, so the instances still running always hold a prefix of
-  // the tiles.  rem[k] / end_tile[k]: member k's remaining steps and the
-  // end of its tiles; launched: steps enqueued since the repack; the graphs
-  // launch grid_tiles blocks, shrunk as the running prefix shrinks.
-  std::vector<int> rem, end_tile;
-  long long launched = 0;
-  int grid_tiles = 0;
-  // tiles of the members still running after s more steps
-  int active_tiles(long long s) const {
-    int t = 0;
-    for (size_t k = 0; k < rem.size() && rem[k] > s; ++k) t = end_tile[k];
-    return t;
-  }
+  GroupKey key{};
+  hipStream_t stream = nullptr;
   Arena ar;
+  int cap_slots = 0, maxj = 1;
+  FreeSpace fs;                 // tiles of the arena not held by placed instances
+  int next_slot = 1;            // slots [1, next_slot) are taken; slot 0 is the dead slot
+  std::vector<int> placed;      // instances in the arena
+  std::vector<int> waiting;     // added, not yet placed
+  int grid_tiles = 0;           // tiles the captured graphs cover
   DevBuf<double> Dt, Lt, w;
   DevBuf<double> Dn, dpart, dout;  // diagnostics
   DevBuf<float> Aop;
   DevBuf<int> fmap;
-  // replayed HIP graphs of kGraphSteps steps ([0]) and of one step ([1])
+  // replayed HIP graphs of kGraphSteps steps ([0]) and of one step ([1]);
+  // graphs replaced while their launches may still run wait in `retired`
+  // until the group's stream has finished that work
   static constexpr int kGraphSteps = 8;
   StepGraph sg[2];
-  GroupKey key{};
+  std::vector<StepGraph> retired;
+  // pinned mirrors of the device step counters and failure flags, copied
+  // asynchronously at the end of every step() call (two buffers, so the
+  // copy of one call and the host's read of the previous one never meet)
+  int *h_cnt = nullptr;  // [2][2 * h_cap]: steps, then failure flags
+  int h_cap = 0;
+  int h_n[2] = {0, 0};   // slots each buffer holds
+  hipEvent_t cev[2] = {nullptr, nullptr};
+  long long calls = 0;   // step() calls that recorded counters since the last compaction
+  long long seen = 0;    // counters of calls [0, seen) have been read
 
-  ~BGroup() { drop_graph(); }
-  void drop_graph() {
+  ~BGroup() {
+    if (stream) (void)hipStreamSynchronize(stream);
     for (auto &e : sg) e.reset();
+    for (auto &r : retired) r.reset();
+    for (auto &e : cev)
+      if (e) (void)hipEventDestroy(e);
+    if (h_cnt) (void)hipHostFree(h_cnt);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void retire_graphs() {
+    for (auto &e : sg)
+      if (e.exec) {
+        retired.push_back(e);
+        e = StepGraph{};
+      }
   }
   size_t bytes() const {
     return ar.bytes() + Dn.bytes() + dpart.bytes() + dout.bytes() + Dt.bytes() + Lt.bytes() + w.bytes() +
@@ -130,8 +227,8 @@ struct BGroup {
   }
 };
 
-// Device bytes one instance adds to its group's arena (exact: the same
-// tables the repack builds).
+// Device bytes one instance adds to its group's arena (exact: the tables
+// its placement writes, with the instance's own job slots per tile).
 size_t instance_bytes(const ElemTables &t, int NP, int NC, int NFP, int dim, int prec, bool limiter) {
   const size_t slots = (size_t)t.til.slots(), tiles = (size_t)t.til.ntiles, js = tiles * t.fj.maxj;
   const size_t word = prec == MXM_PREC_FP64 ? sizeof(double) : sizeof(float);
@@ -140,25 +237,90 @@ size_t instance_bytes(const ElemTables &t, int NP, int NC, int NFP, int dim, int
   b += slots * (dim + 1) * sizeof(int4);                            // limiter neighbours
   b += js * (sizeof(int4) + 4 * sizeof(double) + NFP * sizeof(unsigned short));  // jobs, geometry, partner nodes
   b += tiles * 2 * sizeof(int);                                     // tile_count, tile_inst
-  b += (MXM_PRM_MAX + 2) * sizeof(double) + 3 * sizeof(int);        // per-instance scalars
+  b += (MXM_PRM_MAX + 2) * sizeof(double) + 4 * sizeof(int);        // per-instance scalars
   if (limiter) b += slots * NC * sizeof(double);                    // cell means
   return b;
 }
 
+// Host images of one instance's tables at its place in an arena (base
+// tile bt, slot sl, maxj job slots per tile), in the arena's layouts.
+struct InstImage {
+  std::vector<double> ij, iv, geo, prm;
+  std::vector<int4> cn, jb;
+  std::vector<unsigned short> pr;
+  std::vector<int> tc, ti;
+  double dt = 0.0, post = 1.0;
+  int active = 0, steps = 0, nsteps = 0;
+};
+
+InstImage inst_image(const BGroup &g, const BInst &in, int bt, int sl, int maxj) {
+  InstImage m;
+  const ElemTables &t = in.t;
+  const int epb = g.epb, dd = g.dim * g.dim, NF = g.NF, NFP = g.NFP;
+  const int lt = t.til.ntiles, lmj = t.fj.maxj, bs = bt * epb;
+  const size_t slots = (size_t)lt * epb;
+  m.ij.assign(t.invJ.begin(), t.invJ.begin() + slots * dd);
+  m.iv.assign(t.inv_vol.begin(), t.inv_vol.begin() + slots);
+  m.cn.resize(slots * NF);
+  for (size_t s = 0; s < slots * NF; ++s) {
+    int4 c = t.conn[s];
+    c.x += bs;
+    m.cn[s] = c;
+  }
+  m.jb.resize((size_t)lt * maxj);
+  m.geo.assign((size_t)lt * maxj * 4, 0.0);
+  m.pr.assign((size_t)lt * maxj * NFP, 0);
+  m.tc.resize(lt);
+  m.ti.assign(lt, sl);
+  for (int T = 0; T < lt; ++T) {
+    m.tc[T] = t.til.count[T];
+    for (int j = 0; j < maxj; ++j) {
+      const size_t dj = (size_t)T * maxj + j;
+      if (j < lmj) {
+        const size_t sj = (size_t)T * lmj + j;
+        m.jb[dj] = int4{t.fj.jobs[sj * 4], t.fj.jobs[sj * 4 + 1] + bs, t.fj.jobs[sj * 4 + 2], t.fj.jobs[sj * 4 + 3]};
+        for (int c = 0; c < 4; ++c) m.geo[dj * 4 + c] = t.fj.geo[sj * 4 + c];
+        for (int r = 0; r < NFP; ++r) m.pr[dj * NFP + r] = t.fj.ipr[sj * NFP + r];
+      } else {
+        m.jb[dj] = int4{-1, (bt + T) * epb, 0, 0};  // padding job: in-bounds partner, skipped
+      }
+    }
+  }
+  m.prm.assign(in.prm, in.prm + MXM_PRM_MAX);
+  m.dt = in.dt;
+  m.post = post_factor(in.phys, in.prm, in.dt);
+  m.steps = in.steps_done;
+  m.nsteps = in.nsteps;
+  m.active = (!in.failed && in.steps_done < in.nsteps) ? 1 : 0;
+  return m;
+}
+
+// the instance's state in device order, zero-filled padding slots
+template <class R>
+std::vector<R> device_state(const BGroup &g, const BInst &in) {
+  const size_t node = (size_t)g.NP * g.NC, nv = (size_t)in.ntiles() * g.epb * node;
+  std::vector<R> d(nv, R(0));
+  for (int e = 0; e < in.K; ++e)
+    for (size_t c = 0; c < node; ++c) d[(size_t)in.t.til.dev_of[e] * node + c] = R(in.q0[(size_t)e * node + c]);
+  return d;
+}
+
 struct Batch {
   int prec = MXM_PREC_FP64;
-  hipStream_t stream = nullptr;
+  hipStream_t cstream = nullptr;  // graph capture only (never runs work)
   std::map<GroupKey, std::unique_ptr<BGroup>> groups;
   std::map<int, std::unique_ptr<BInst>> insts;
   std::map<std::pair<int, int>, RefElem> refs;
+  std::vector<std::unique_ptr<Staging>> staging;
   int next_id = 0;
-  // autotuned launch configuration per (group, tile-count octave)
+  // autotuned launch configuration per (group, grid-size octave)
   bool tune = true;
   std::map<std::pair<GroupKey, int>, int> tuned;
 
   ~Batch() {
-    groups.clear();
-    if (stream) (void)hipStreamDestroy(stream);
+    groups.clear();  // drains each group's stream
+    staging.clear();
+    if (cstream) (void)hipStreamDestroy(cstream);
   }
 
   const RefElem &ref(int dim, int P) {
@@ -168,10 +330,16 @@ struct Batch {
     return it->second;
   }
 
-  BGroup &group_of(const BInst &in) {
+  BInst &inst(int id) {
+    auto it = insts.find(id);
+    if (it == insts.end()) throw ArgError("unknown instance id");
+    return *it->second;
+  }
+
+  BGroup &group_of_key(const BInst &in) {
     long long eb = 0;
     std::memcpy(&eb, &in.eps, sizeof(eb));
-    GroupKey key{in.dim, in.P, in.phys, in.variant, in.limiter ? 1 : 0, in.limiter ? eb : 0};
+    const GroupKey key{in.dim, in.P, in.phys, in.variant, in.limiter ? 1 : 0, in.limiter ? eb : 0};
     auto it = groups.find(key);
     if (it != groups.end()) return *it->second;
     auto g = std::make_unique<BGroup>();
@@ -195,7 +363,19 @@ struct Batch {
     g->w.upload(re.w.data(), re.w.size());
     g->Dn.upload(re.Dn.data(), re.Dn.size());
     g->fmap.upload(re.fmap.data(), re.fmap.size());
+    MXM_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    for (auto &e : g->cev) MXM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return *groups.emplace(key, std::move(g)).first->second;
+  }
+
+  BGroup &group_with(int id) {
+    const BInst &in = inst(id);
+    long long eb = 0;
+    std::memcpy(&eb, &in.eps, sizeof(eb));
+    const GroupKey key{in.dim, in.P, in.phys, in.variant, in.limiter ? 1 : 0, in.limiter ? eb : 0};
+    auto it = groups.find(key);
+    if (it == groups.end()) throw ArgError("instance has no group");
+    return *it->second;
   }
 
   int add(int dim, const void *desc, int P, int phys, const double *prm, double dt, int nsteps, int limiter,
@@ -228,145 +408,235 @@ struct Batch {
     in->q0.assign(q0, q0 + n);
     in->t = build_elem_tables(m, re, nc);
     in->id = next_id++;
-    BGroup &g = group_of(*in);
-    g.members.push_back(in->id);
-    g.dirty = true;
+    BGroup &g = group_of_key(*in);
+    g.waiting.push_back(in->id);
     const int id = in->id;
     insts.emplace(id, std::move(in));
     return id;
   }
 
-  BInst &inst(int id) {
-    auto it = insts.find(id);
-    if (it == insts.end()) throw ArgError("unknown instance id");
-    return *it->second;
+  // ---- device step counters ---------------------------------------------
+  void ensure_counter_mirror(BGroup &g) {
+    if (g.h_cap >= g.cap_slots && g.h_cnt) return;
+    if (g.h_cnt) (void)hipHostFree(g.h_cnt);
+    g.h_cnt = nullptr;
+    g.h_cap = std::max(g.cap_slots, 16);
+    MXM_HIP(hipHostMalloc(reinterpret_cast<void **>(&g.h_cnt), (size_t)2 * 2 * g.h_cap * sizeof(int)));
   }
 
-  BGroup &group_with(int id) {
-    for (auto &kv : groups)
-      for (int m : kv.second->members)
-        if (m == id) return *kv.second;
-    throw ArgError("instance has no group");
+  // host mirrors from counter buffer b (valid: its copy has completed)
+  void absorb_counters(BGroup &g, int b) {
+    const int *st = g.h_cnt + (size_t)b * 2 * g.h_cap, *fl = st + g.h_cap;
+    for (int id : g.placed) {
+      BInst &in = inst(id);
+      if (in.slot >= g.h_n[b]) continue;  // placed after that copy
+      in.steps_done = std::max(in.steps_done, st[in.slot]);
+      if (fl[in.slot]) in.failed = true;
+    }
   }
 
-  // host mirrors of the device step counters of the packed arena
+  // synchronous read of the counters of every placed instance (the
+  // group's stream must be idle)
   void pull_counters(BGroup &g) {
-    if (g.packed.empty() || !g.ar.isteps.p) return;
-    std::vector<int> st(g.packed.size());
+    if (g.placed.empty() || !g.ar.isteps.p) return;
+    std::vector<int> st(g.next_slot), fl(g.next_slot);
     MXM_HIP(hipMemcpy(st.data(), g.ar.isteps.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < g.packed.size(); ++k) {
-      auto it = insts.find(g.packed[k]);
-      if (it != insts.end() && it->second->placed) it->second->steps_done = st[k];
+    MXM_HIP(hipMemcpy(fl.data(), g.ar.ifail.p, fl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (int id : g.placed) {
+      BInst &in = inst(id);
+      in.steps_done = st[in.slot];
+      if (fl[in.slot]) in.failed = true;
     }
   }
 
-  // The new arena's state trio: placed members' tiles copied from the old
-  // arena (their old base tiles are still in BInst::base_tile), new members
-  // uploaded from q0.
+  void drain(BGroup &g) {
+    MXM_HIP(hipStreamSynchronize(g.stream));
+    for (auto &r : g.retired) r.reset();
+    g.retired.clear();
+  }
+
+  // ---- staging ----------------------------------------------------------
+  Staging &stage_buf(size_t bytes) {
+    for (auto &s : staging)
+      if (s->busy && hipEventQuery(s->ev) == hipSuccess) s->busy = false;
+    for (auto &s : staging)
+      if (!s->busy && s->cap >= bytes) return *s;
+    auto s = std::make_unique<Staging>();
+    s->cap = std::max<size_t>(bytes, 1 << 20);
+    MXM_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->p), s->cap));
+    MXM_HIP(hipEventCreateWithFlags(&s->ev, hipEventDisableTiming));
+    staging.push_back(std::move(s));
+    return *staging.back();
+  }
+
+  // ---- placement --------------------------------------------------------
+  // asynchronous upload of one instance at (bt, sl) into the arena
   template <class R>
-  void place_states(const BGroup &g, DevBuf<R> (&q)[3], DevBuf<R> (&nq)[3], const std::vector<int> &order,
-                    const std::vector<int> &new_base) {
-    const size_t tile_vals = (size_t)g.epb * g.NP * g.NC;
-    for (auto &b : nq) {
-      b.alloc(std::max<size_t>(1, (size_t)g.ntiles * tile_vals));
-      MXM_HIP(hipMemset(b.p, 0, b.bytes()));
+  void upload_instance(BGroup &g, DevBuf<R> (&q)[3], BInst &in, int bt, int sl) {
+    const InstImage m = inst_image(g, in, bt, sl, g.maxj);
+    const std::vector<R> qs = device_state<R>(g, in);
+    const int dd = g.dim * g.dim, NF = g.NF;
+    const size_t bs = (size_t)bt * g.epb;
+    struct Seg {
+      void *dst;
+      const void *src;
+      size_t bytes;
+    };
+    const Seg segs[] = {
+        {q[0].p + bs * g.NP * g.NC, qs.data(), qs.size() * sizeof(R)},
+        {g.ar.invJ.p + bs * dd, m.ij.data(), m.ij.size() * sizeof(double)},
+        {g.ar.inv_vol.p + bs, m.iv.data(), m.iv.size() * sizeof(double)},
+        {g.ar.conn.p + bs * NF, m.cn.data(), m.cn.size() * sizeof(int4)},
+        {g.ar.jobs.p + (size_t)bt * g.maxj, m.jb.data(), m.jb.size() * sizeof(int4)},
+        {g.ar.jgeo.p + (size_t)bt * g.maxj * 4, m.geo.data(), m.geo.size() * sizeof(double)},
+        {g.ar.ipr.p + (size_t)bt * g.maxj * g.NFP, m.pr.data(), m.pr.size() * sizeof(unsigned short)},
+        {g.ar.tile_count.p + bt, m.tc.data(), m.tc.size() * sizeof(int)},
+        {g.ar.tile_inst.p + bt, m.ti.data(), m.ti.size() * sizeof(int)},
+        {g.ar.iprm.p + (size_t)sl * MXM_PRM_MAX, m.prm.data(), MXM_PRM_MAX * sizeof(double)},
+        {g.ar.idt.p + sl, &m.dt, sizeof(double)},
+        {g.ar.ipost.p + sl, &m.post, sizeof(double)},
+        {g.ar.isteps.p + sl, &m.steps, sizeof(int)},
+        {g.ar.insteps.p + sl, &m.nsteps, sizeof(int)},
+        {g.ar.iactive.p + sl, &m.active, sizeof(int)},
+    };
+    size_t total = 0;
+    for (const Seg &s : segs) total += (s.bytes + 15) / 16 * 16;
+    Staging &st = stage_buf(total);
+    size_t off = 0;
+    for (const Seg &s : segs) {
+      std::memcpy(st.p + off, s.src, s.bytes);
+      MXM_HIP(hipMemcpyAsync(s.dst, st.p + off, s.bytes, hipMemcpyHostToDevice, g.stream));
+      off += (s.bytes + 15) / 16 * 16;
     }
-    for (size_t k = 0; k < order.size(); ++k) {
-      const BInst &in = inst(order[k]);
-      R *dst = nq[0].p + (size_t)new_base[k] * tile_vals;
-      const size_t nv = (size_t)in.t.til.ntiles * tile_vals;
-      if (in.placed) {
-        MXM_HIP(hipMemcpy(dst, q[0].p + (size_t)in.base_tile * tile_vals, nv * sizeof(R), hipMemcpyDeviceToDevice));
-      } else {
-        const size_t node = (size_t)g.NP * g.NC;
-        std::vector<R> d(nv, R(0));
-        for (int e = 0; e < in.K; ++e)
-          for (size_t c = 0; c < node; ++c) d[(size_t)in.t.til.dev_of[e] * node + c] = R(in.q0[(size_t)e * node + c]);
-        MXM_HIP(hipMemcpy(dst, d.data(), nv * sizeof(R), hipMemcpyHostToDevice));
-      }
-    }
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, g.stream, g.ar.ifail.p + sl, 0, 1);
+    MXM_HIP(hipGetLastError());
+    MXM_HIP(hipEventRecord(st.ev, g.stream));
+    st.busy = true;
   }
 
-  // Rebuilds the group's arena for its current member list.  Everything is
-  // built in a new arena first; only when every allocation and copy has
-  // succeeded are the arena, the member order, the base tiles and the
-  // placed flags replaced, so an error leaves the group as it was (still
-  // dirty, to be retried).
-  void repack(BGroup &g) {
-    MXM_HIP(hipStreamSynchronize(stream));
-    pull_counters(g);
-    g.drop_graph();
-    std::vector<int> order = g.members;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-      const BInst &x = inst(a), &y = inst(b);
-      return x.nsteps - x.steps_done > y.nsteps - y.steps_done;
-    });
-    std::vector<int> new_base(order.size());
-    int tiles = 0, maxj = 1;
-    for (size_t k = 0; k < order.size(); ++k) {
-      const BInst &in = inst(order[k]);
-      new_base[k] = tiles;
-      tiles += in.t.til.ntiles;
-      maxj = std::max(maxj, in.t.fj.maxj);
+  // places the group's waiting instances into free ranges (asynchronous);
+  // compacts the group when they do not all fit
+  void place(BGroup &g) {
+    if (g.waiting.empty()) return;
+    std::vector<int> order = g.waiting;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst(a).nsteps > inst(b).nsteps; });
+    // fits without a compaction?
+    bool fits = g.fs.cap > 0 && g.next_slot + (int)order.size() <= g.cap_slots;
+    if (fits) {
+      FreeSpace trial = g.fs;
+      for (int id : order) {
+        const BInst &in = inst(id);
+        const int r = in.t.fj.maxj <= g.maxj ? trial.find(in.ntiles()) : -1;
+        if (r < 0) {
+          fits = false;
+          break;
+        }
+        trial.take(r, in.ntiles());
+      }
     }
-    const int old_ntiles = g.ntiles;
-    g.ntiles = tiles;  // place_states sizes the new trio from it
-    Arena na;
-    try {
+    if (!fits) {
+      compact(g);
+      return;
+    }
+    for (int id : order) {
+      BInst &in = inst(id);
+      const int bt = g.fs.find(in.ntiles());
+      g.fs.take(bt, in.ntiles());
+      const int sl = g.next_slot++;
       if (g.prec == MXM_PREC_FP64)
-        place_states(g, g.ar.qd, na.qd, order, new_base);
+        upload_instance(g, g.ar.qd, in, bt, sl);
       else
-        place_states(g, g.ar.qf, na.qf, order, new_base);
-    } catch (...) {
-      g.ntiles = old_ntiles;
-      throw;
+        upload_instance(g, g.ar.qf, in, bt, sl);
+      in.base_tile = bt, in.slot = sl, in.placed = true;
+      in.steps_enqueued = in.steps_done;
+      in.q0.clear();
+      in.q0.shrink_to_fit();
+      g.placed.push_back(id);
     }
-    g.ntiles = old_ntiles;
-    // concatenated tables, slots offset by each instance's base
+    g.waiting.clear();
+  }
+
+  // Rebuilds the group's arena for its live and waiting instances, sized
+  // exactly for them.  Everything is built in a new arena first; only when
+  // every allocation and copy has succeeded are the arena and the
+  // instances' places replaced, so an error leaves the group as it was
+  // (its waiting instances still waiting).
+  void compact(BGroup &g) {
+    drain(g);
+    pull_counters(g);
+    // every placed instance stays (finished ones remain readable until
+    // they are removed), longest remaining run first: finished and failed
+    // instances end up in the tail, which the grid leaves out
+    std::vector<int> order = g.placed;
+    order.insert(order.end(), g.waiting.begin(), g.waiting.end());
+    auto remaining = [&](int id) {
+      const BInst &x = inst(id);
+      return x.failed ? 0 : std::max(0, x.nsteps - x.steps_done);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return remaining(a) > remaining(b); });
+    int tiles = 0, maxj = 1;
+    for (int id : order) {
+      tiles += inst(id).ntiles();
+      maxj = std::max(maxj, inst(id).t.fj.maxj);
+    }
+    const int ni = (int)order.size(), cap_slots = ni + 1;
     const int epb = g.epb, dd = g.dim * g.dim, NF = g.NF, NFP = g.NFP;
-    const size_t slots = (size_t)tiles * epb;
-    std::vector<double> ij(slots * dd, 0.0), iv(slots, 0.0), geo((size_t)tiles * maxj * 4, 0.0);
-    std::vector<int4> cn(slots * NF), jb((size_t)tiles * maxj);
-    std::vector<unsigned short> pr((size_t)tiles * maxj * NFP, 0);
-    std::vector<int> tc(tiles, 0), ti(tiles, 0);
-    const size_t ni = order.size();
-    std::vector<double> iprm(ni * MXM_PRM_MAX), idt(ni), ipost(ni);
-    std::vector<int> iact(ni), ist(ni), inst_n(ni);
-    for (size_t k = 0; k < ni; ++k) {
+    const size_t slots = (size_t)std::max(tiles, 1) * epb, tile_vals = (size_t)epb * g.NP * g.NC;
+    Arena na;
+    auto state_trio = [&](auto &nq, auto &oq) {
+      using R = typename std::remove_reference<decltype(*nq[0].p)>::type;
+      for (auto &b : nq) {
+        b.alloc(slots * g.NP * g.NC);
+        MXM_HIP(hipMemset(b.p, 0, b.bytes()));
+      }
+      int bt = 0;
+      for (int id : order) {
+        const BInst &in = inst(id);
+        R *dst = nq[0].p + (size_t)bt * tile_vals;
+        const size_t nv = (size_t)in.ntiles() * tile_vals;
+        if (in.placed) {
+          MXM_HIP(hipMemcpy(dst, oq[0].p + (size_t)in.base_tile * tile_vals, nv * sizeof(R),
+                            hipMemcpyDeviceToDevice));
+        } else {
+          const std::vector<R> d = device_state<R>(g, in);
+          MXM_HIP(hipMemcpy(dst, d.data(), nv * sizeof(R), hipMemcpyHostToDevice));
+        }
+        bt += in.ntiles();
+      }
+    };
+    if (g.prec == MXM_PREC_FP64)
+      state_trio(na.qd, g.ar.qd);
+    else
+      state_trio(na.qf, g.ar.qf);
+    std::vector<double> ij(slots * dd, 0.0), iv(slots, 0.0), geo((size_t)std::max(tiles, 1) * maxj * 4, 0.0);
+    std::vector<int4> cn(slots * NF), jb((size_t)std::max(tiles, 1) * maxj);
+    std::vector<unsigned short> pr((size_t)std::max(tiles, 1) * maxj * NFP, 0);
+    std::vector<int> tc(std::max(tiles, 1), 0), ti(std::max(tiles, 1), 0);
+    std::vector<double> iprm((size_t)cap_slots * MXM_PRM_MAX, 0.0), idt(cap_slots, 0.0), ipost(cap_slots, 1.0);
+    std::vector<int> iact(cap_slots, 0), ist(cap_slots, 0), inst_n(cap_slots, 0), ifl(cap_slots, 0);
+    for (int s = 0; s < (int)slots; ++s)
+      for (int lf = 0; lf < NF; ++lf) cn[(size_t)s * NF + lf] = int4{s, 0, 0, 0};
+    for (int T = 0; T < std::max(tiles, 1); ++T)
+      for (int j = 0; j < maxj; ++j) jb[(size_t)T * maxj + j] = int4{-1, T * epb, 0, 0};
+    std::vector<int> new_base(ni);
+    int bt = 0;
+    for (int k = 0; k < ni; ++k) {
       const BInst &in = inst(order[k]);
-      const ElemTables &t = in.t;
-      const int bt = new_base[k], bs = bt * epb, lt = t.til.ntiles, lmj = t.fj.maxj;
-      for (int s = 0; s < lt * epb; ++s) {
-        for (int q = 0; q < dd; ++q) ij[(size_t)(bs + s) * dd + q] = t.invJ[(size_t)s * dd + q];
-        iv[bs + s] = t.inv_vol[s];
-        for (int lf = 0; lf < NF; ++lf) {
-          int4 c = t.conn[(size_t)s * NF + lf];
-          c.x += bs;
-          cn[(size_t)(bs + s) * NF + lf] = c;
-        }
-      }
-      for (int T = 0; T < lt; ++T) {
-        tc[bt + T] = t.til.count[T];
-        ti[bt + T] = (int)k;
-        for (int j = 0; j < maxj; ++j) {
-          const size_t dj = (size_t)(bt + T) * maxj + j;
-          if (j < lmj) {
-            const size_t sj = (size_t)T * lmj + j;
-            int4 r{t.fj.jobs[sj * 4], t.fj.jobs[sj * 4 + 1] + bs, t.fj.jobs[sj * 4 + 2], t.fj.jobs[sj * 4 + 3]};
-            jb[dj] = r;
-            for (int c = 0; c < 4; ++c) geo[dj * 4 + c] = t.fj.geo[sj * 4 + c];
-            for (int r2 = 0; r2 < NFP; ++r2) pr[dj * NFP + r2] = t.fj.ipr[sj * NFP + r2];
-          } else {
-            jb[dj] = int4{-1, (bt + T) * epb, 0, 0};
-          }
-        }
-      }
-      std::memcpy(&iprm[k * MXM_PRM_MAX], in.prm, sizeof(in.prm));
-      idt[k] = in.dt;
-      ipost[k] = post_factor(in.phys, in.prm, in.dt);
-      ist[k] = in.steps_done;
-      inst_n[k] = in.nsteps;
-      iact[k] = in.steps_done < in.nsteps ? 1 : 0;
+      const int sl = k + 1;
+      const InstImage m = inst_image(g, in, bt, sl, maxj);
+      const size_t bs = (size_t)bt * epb;
+      std::copy(m.ij.begin(), m.ij.end(), ij.begin() + bs * dd);
+      std::copy(m.iv.begin(), m.iv.end(), iv.begin() + bs);
+      std::copy(m.cn.begin(), m.cn.end(), cn.begin() + bs * NF);
+      std::copy(m.jb.begin(), m.jb.end(), jb.begin() + (size_t)bt * maxj);
+      std::copy(m.geo.begin(), m.geo.end(), geo.begin() + (size_t)bt * maxj * 4);
+      std::copy(m.pr.begin(), m.pr.end(), pr.begin() + (size_t)bt * maxj * NFP);
+      std::copy(m.tc.begin(), m.tc.end(), tc.begin() + bt);
+      std::copy(m.ti.begin(), m.ti.end(), ti.begin() + bt);
+      std::copy(m.prm.begin(), m.prm.end(), iprm.begin() + (size_t)sl * MXM_PRM_MAX);
+      idt[sl] = m.dt, ipost[sl] = m.post, iact[sl] = m.active, ist[sl] = m.steps, inst_n[sl] = m.nsteps;
+      new_base[k] = bt;
+      bt += in.ntiles();
     }
     na.invJ.upload(ij.data(), ij.size());
     na.inv_vol.upload(iv.data(), iv.size());
@@ -382,39 +652,47 @@ struct Batch {
     na.iactive.upload(iact.data(), iact.size());
     na.isteps.upload(ist.data(), ist.size());
     na.insteps.upload(inst_n.data(), inst_n.size());
+    na.ifail.upload(ifl.data(), ifl.size());
     if (g.limiter) na.mean.alloc(slots * g.NC);
     // ---- commit (no more allocations or copies that can fail) ----
     g.ar.swap(na);
-    g.ntiles = tiles;
-    g.maxj = maxj;
-    g.members = order;
-    g.packed = order;
-    for (size_t k = 0; k < ni; ++k) {
+    for (auto &e : g.sg) e.reset();
+    g.grid_tiles = 0;
+    g.cap_slots = cap_slots, g.maxj = maxj;
+    g.fs = FreeSpace{};
+    g.fs.top = g.fs.cap = tiles;
+    g.next_slot = cap_slots;
+    for (int k = 0; k < ni; ++k) {
       BInst &in = inst(order[k]);
-      in.base_tile = new_base[k];
-      if (!in.placed) {
-        in.placed = true;
+      in.base_tile = new_base[k], in.slot = k + 1, in.placed = true;
+      in.steps_enqueued = in.steps_done;
+      if (!in.q0.empty()) {
         in.q0.clear();
         in.q0.shrink_to_fit();
       }
     }
-    g.dirty = false;
-    g.rem.resize(ni);
-    g.end_tile.resize(ni);
-    for (size_t k = 0; k < ni; ++k) {
-      const BInst &in = inst(order[k]);
-      g.rem[k] = std::max(0, in.nsteps - in.steps_done);
-      g.end_tile[k] = in.base_tile + in.t.til.ntiles;
-    }
-    g.launched = 0;
-    recapture(g, g.active_tiles(0));
+    g.placed = order;
+    g.waiting.clear();
+    g.calls = g.seen = 0;
+    ensure_counter_mirror(g);
   }
 
-  // graphs over the first `tiles` tiles of the arena (none when 0); the
-  // old graphs finish before they are destroyed
+  // ---- stepping ---------------------------------------------------------
+  // one past the last tile of the instances still to run at least one of
+  // the next steps
+  int live_top(const BGroup &g) {
+    int t = 0;
+    for (int id : g.placed) {
+      const BInst &in = inst(id);
+      if (in.live()) t = std::max(t, in.base_tile + in.ntiles());
+    }
+    return t;
+  }
+
+  // graphs over the first `tiles` tiles (none when 0); the old ones are
+  // retired until the work launched from them is done
   void recapture(BGroup &g, int tiles) {
-    MXM_HIP(hipStreamSynchronize(stream));
-    g.drop_graph();
+    g.retire_graphs();
     g.grid_tiles = tiles;
     if (tiles > 0) {
       if (tune) choose_config(g);
@@ -422,25 +700,23 @@ struct Batch {
     }
   }
 
-  // the launch configuration for the group's packed size: from the cache
-  // of this (group, tile-count octave), else timed on stage 1 (q -> q1,
-  // which the next step rewrites before reading)
+  // the launch configuration for the group's grid size: from the cache of
+  // this (group, tile-count octave), else timed on stage 1 (q -> q1, which
+  // the next step rewrites before reading)
   void choose_config(BGroup &g) {
     int oct = 0;
     while ((1 << oct) < g.grid_tiles) ++oct;
     const auto key = std::make_pair(g.key, oct);
     auto it = tuned.find(key);
     if (it == tuned.end()) {
-      MXM_HIP(hipStreamSynchronize(stream));
       int c;
       if (g.prec == MXM_PREC_FP64) {
         const StageArgs<double> a = stage_args(g, g.ar.qd, 0);
-        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, stream, 10, nullptr);
+        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr);
       } else {
         const StageArgs<float> a = stage_args(g, g.ar.qf, 0);
-        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, stream, 10, nullptr);
+        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr);
       }
-      MXM_HIP(hipStreamSynchronize(stream));
       it = tuned.emplace(key, c).first;
     }
     g.cfg = it->second;
@@ -456,6 +732,9 @@ struct Batch {
     a.a = pl.a, a.b = pl.b, a.c = pl.c;
     a.tile_inst = ar.tile_inst.p, a.inst_prm = ar.iprm.p, a.inst_dt = ar.idt.p, a.inst_post = ar.ipost.p;
     a.inst_active = ar.iactive.p;
+    a.inst_fail = ar.ifail.p;
+    a.mean = g.limiter ? ar.mean.p : nullptr;
+    a.w = g.w.p;
     a.dt = 0.0, a.post_scale = 1.0;
     // GLM decay per instance (factor 1 where alpha_d = 0)
     a.post_comp = s == 2 && g.phys == MXM_PHYS_MHDGLM2D ? 6 : -1;
@@ -467,40 +746,39 @@ struct Batch {
   }
 
   template <class R>
-  void launch_step(BGroup &g, DevBuf<R> (&q)[3]) {
-    const int ni = (int)g.members.size();
-    for (int s = 0; s < 3; ++s) {
-      const StageArgs<R> a = stage_args(g, q, s);
-      MXM_HIP(g.launch(&a, stream));
+  void launch_step(BGroup &g, DevBuf<R> (&q)[3], hipStream_t s) {
+    for (int st = 0; st < 3; ++st) {
+      const StageArgs<R> a = stage_args(g, q, st);
+      MXM_HIP(g.launch(&a, s));
       if (g.limiter)
-        MXM_HIP(launch_limiter<R>(a.q_out, g.ar.mean.p, g.w.p, g.ar.conn.p, g.grid_tiles * g.epb, g.NP, g.NC, g.NF,
-                                  g.dim, g.eps, stream, g.ar.tile_inst.p, g.ar.iactive.p, g.epb));
+        MXM_HIP(launch_bj_apply<R>(a.q_out, g.ar.mean.p, g.ar.conn.p, g.grid_tiles * g.epb, g.NP, g.NC, g.NF, g.dim,
+                                   g.eps, s, g.ar.tile_count.p, g.epb, g.ar.tile_inst.p, g.ar.iactive.p));
     }
-    hipLaunchKernelGGL(bump_steps_kernel, dim3((ni + 255) / 256), dim3(256), 0, stream, g.ar.isteps.p,
-                       g.ar.insteps.p, g.ar.iactive.p, ni);
+    // over every slot of the arena (later placements need no recapture;
+    // slots not in use are inactive)
+    hipLaunchKernelGGL(bump_steps_kernel, dim3((g.cap_slots + 255) / 256), dim3(256), 0, s, g.ar.isteps.p,
+                       g.ar.insteps.p, g.ar.iactive.p, g.ar.ifail.p, g.cap_slots);
     MXM_HIP(hipGetLastError());
   }
 
-  void launch_step(BGroup &g) {
-    if (g.prec == MXM_PREC_FP64)
-      launch_step(g, g.ar.qd);
-    else
-      launch_step(g, g.ar.qf);
-  }
-
-  // n SSPRK3 steps of the group as one HIP graph (kernel nodes only); the
-  // graph is kept beside its executable
+  // n SSPRK3 steps of the group as one HIP graph (kernel nodes only),
+  // captured on the capture stream; the graph is kept beside its executable
   void capture_steps(BGroup &g, int n, StepGraph &out) {
-    MXM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    MXM_HIP(hipStreamBeginCapture(cstream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int k = 0; k < n; ++k) launch_step(g);
+      for (int k = 0; k < n; ++k) {
+        if (g.prec == MXM_PREC_FP64)
+          launch_step(g, g.ar.qd, cstream);
+        else
+          launch_step(g, g.ar.qf, cstream);
+      }
     } catch (...) {
       hipGraph_t tmp = nullptr;
-      (void)hipStreamEndCapture(stream, &tmp);
+      (void)hipStreamEndCapture(cstream, &tmp);
       if (tmp) (void)hipGraphDestroy(tmp);
       throw;
     }
-    MXM_HIP(hipStreamEndCapture(stream, &out.graph));
+    MXM_HIP(hipStreamEndCapture(cstream, &out.graph));
     MXM_HIP(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
   }
   void capture(BGroup &g) {
@@ -508,51 +786,116 @@ struct Batch {
     capture_steps(g, 1, g.sg[1]);
   }
 
-  void step(int n) {
-    for (auto &kv : groups)
-      if (kv.second->dirty) repack(*kv.second);
-    // instances that finish inside a launch are switched off by their
-    // device counters; the grid shrinks to the running prefix between
-    // launches once it has lost an eighth of its tiles
-    auto launch_all = [&](int which, int k) {
-      for (auto &kv : groups) {
-        BGroup &g = *kv.second;
-        const int need = g.active_tiles(g.launched);
-        g.launched += k;
-        if (need == 0) continue;
-        if (need <= g.grid_tiles - g.grid_tiles / 8) recapture(g, need);
-        MXM_HIP(hipGraphLaunch(g.sg[which].exec, stream));
-      }
-    };
-    for (; n >= BGroup::kGraphSteps; n -= BGroup::kGraphSteps) launch_all(0, BGroup::kGraphSteps);
-    for (; n > 0; --n) launch_all(1, 1);
+  // k steps of the group from graph `which` (k = its step count)
+  void launch_graph(BGroup &g, int which, int k) {
+    const int need = live_top(g);
+    if (need == 0) return;
+    if (need > g.grid_tiles || need <= g.grid_tiles - g.grid_tiles / 8 || !g.sg[which].exec) recapture(g, need);
+    MXM_HIP(hipGraphLaunch(g.sg[which].exec, g.stream));
+    for (int id : g.placed) {
+      BInst &in = inst(id);
+      if (in.live()) in.steps_enqueued = std::min(in.nsteps, in.steps_enqueued + k);
+    }
   }
 
-  void sync_counters() {
-    MXM_HIP(hipStreamSynchronize(stream));
-    for (auto &kv : groups) pull_counters(*kv.second);
+  // the counters of this call, copied into the pinned mirror behind the
+  // steps just launched
+  void record_counters(BGroup &g) {
+    if (g.placed.empty()) return;
+    ensure_counter_mirror(g);
+    const int b = (int)(g.calls & 1);
+    int *dst = g.h_cnt + (size_t)b * 2 * g.h_cap;
+    MXM_HIP(hipMemcpyAsync(dst, g.ar.isteps.p, (size_t)g.next_slot * sizeof(int), hipMemcpyDeviceToHost, g.stream));
+    MXM_HIP(hipMemcpyAsync(dst + g.h_cap, g.ar.ifail.p, (size_t)g.next_slot * sizeof(int), hipMemcpyDeviceToHost,
+                           g.stream));
+    MXM_HIP(hipEventRecord(g.cev[b], g.stream));
+    g.h_n[b] = g.next_slot;
+    ++g.calls;
+  }
+
+  // releases graphs retired before a call whose work is known done
+  void reap(BGroup &g) {
+    if (g.retired.empty()) return;
+    if (hipStreamQuery(g.stream) != hipSuccess) return;
+    for (auto &r : g.retired) r.reset();
+    g.retired.clear();
+  }
+
+  void step(int n) {
+    for (auto &kv : groups) {
+      BGroup &g = *kv.second;
+      reap(g);
+      place(g);
+      int m = n;
+      for (; m >= BGroup::kGraphSteps; m -= BGroup::kGraphSteps) launch_graph(g, 0, BGroup::kGraphSteps);
+      for (; m > 0; --m) launch_graph(g, 1, 1);
+      record_counters(g);
+    }
+  }
+
+  // Instances newly finished (or failed).  lag = 0 waits for all launched
+  // work; lag = 1 waits only for the work of step() calls before the most
+  // recent one (the most recent keeps the device busy meanwhile).
+  std::vector<int> poll(int lag) {
+    std::vector<int> out;
+    for (auto &kv : groups) {
+      BGroup &g = *kv.second;
+      if (lag <= 0 || g.calls < 2) {
+        drain(g);
+        pull_counters(g);
+      } else {
+        const long long c = g.calls - 2;  // the call before the most recent
+        if (c >= g.seen) {
+          const int b = (int)(c & 1);
+          MXM_HIP(hipEventSynchronize(g.cev[b]));
+          absorb_counters(g, b);
+          g.seen = c + 1;
+        }
+      }
+    }
+    for (auto &kv : insts) {
+      BInst &in = *kv.second;
+      if (!in.reported && in.placed && (in.failed || in.steps_done >= in.nsteps)) {
+        out.push_back(in.id);
+        in.reported = true;
+      }
+    }
+    return out;
+  }
+
+  void sync_all() {
+    for (auto &kv : groups) {
+      drain(*kv.second);
+      pull_counters(*kv.second);
+    }
   }
 
   template <class R>
   void read_state(BGroup &g, DevBuf<R> (&q)[3], BInst &in, double *h) {
     const size_t node = (size_t)g.NP * g.NC, tile_vals = (size_t)g.epb * node;
-    const size_t nv = (size_t)in.t.til.ntiles * tile_vals;
+    const size_t nv = (size_t)in.ntiles() * tile_vals;
     std::vector<R> d(nv);
     MXM_HIP(hipMemcpy(d.data(), q[0].p + (size_t)in.base_tile * tile_vals, nv * sizeof(R), hipMemcpyDeviceToHost));
     for (int e = 0; e < in.K; ++e)
       for (size_t c = 0; c < node; ++c) h[(size_t)e * node + c] = double(d[(size_t)in.t.til.dev_of[e] * node + c]);
   }
 
+  // the group of a placed instance, drained (placement first if needed)
+  BGroup &settled(int id) {
+    BGroup &g = group_with(id);
+    if (!inst(id).placed) place(g);
+    drain(g);
+    return g;
+  }
+
   void get_state(int id, double *h, long long n) {
     BInst &in = inst(id);
     if (!h || n != (long long)in.K * in.NP * in.NC) throw ArgError("state length must be K*NP*NC");
-    if (!in.placed) {
+    if (!in.placed && !in.q0.empty()) {
       std::memcpy(h, in.q0.data(), n * sizeof(double));
       return;
     }
-    BGroup &g = group_with(id);
-    if (g.dirty) repack(g);
-    MXM_HIP(hipStreamSynchronize(stream));
+    BGroup &g = settled(id);
     if (g.prec == MXM_PREC_FP64)
       read_state(g, g.ar.qd, in, h);
     else
@@ -562,9 +905,8 @@ struct Batch {
   // the largest signal speed over the nodes of one instance
   double max_wave_speed(int id) {
     BInst &in = inst(id);
-    BGroup &g = group_with(id);
-    if (g.dirty) repack(g);
-    const int nt = in.t.til.ntiles;
+    BGroup &g = settled(id);
+    const int nt = in.ntiles();
     if (g.dpart.n < (size_t)nt) g.dpart.alloc(nt);
     if (g.dout.n < 1) g.dout.alloc(1);
     auto fill = [&](auto &a) {
@@ -575,25 +917,24 @@ struct Batch {
       SpeedArgs<double> a{};
       a.q = g.ar.qd[0].p;
       fill(a);
-      MXM_HIP(launch_max_speed(g.phys, true, &a, stream));
+      MXM_HIP(launch_max_speed(g.phys, true, &a, g.stream));
     } else {
       SpeedArgs<float> a{};
       a.q = g.ar.qf[0].p;
       fill(a);
-      MXM_HIP(launch_max_speed(g.phys, false, &a, stream));
+      MXM_HIP(launch_max_speed(g.phys, false, &a, g.stream));
     }
     double h = 0.0;
-    MXM_HIP(hipMemcpyAsync(&h, g.dout.p, sizeof(double), hipMemcpyDeviceToHost, stream));
-    MXM_HIP(hipStreamSynchronize(stream));
+    MXM_HIP(hipMemcpyAsync(&h, g.dout.p, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    MXM_HIP(hipStreamSynchronize(g.stream));
     return h;
   }
 
   // diagnostics of one instance (Solver::diagnostics over its tiles)
   void diagnostics(int id, double *h) {
     BInst &in = inst(id);
-    BGroup &g = group_with(id);
-    if (g.dirty) repack(g);
-    const int nt = in.t.til.ntiles, NC = g.NC;
+    BGroup &g = settled(id);
+    const int nt = in.ntiles(), NC = g.NC;
     const size_t per = (size_t)kDiagSums * 2 + kDiagMaxNC;
     if (g.dpart.n < (size_t)nt * per) g.dpart.alloc((size_t)nt * per);
     if (g.dout.n < (size_t)(3 * NC + 2)) g.dout.alloc(3 * NC + 2);
@@ -606,27 +947,49 @@ struct Batch {
       DiagArgs<double> a{};
       a.q = g.ar.qd[0].p;
       fill(a);
-      MXM_HIP(launch_diagnostics(g.dim, g.P, true, &a, stream));
+      MXM_HIP(launch_diagnostics(g.dim, g.P, true, &a, g.stream));
     } else {
       DiagArgs<float> a{};
       a.q = g.ar.qf[0].p;
       fill(a);
-      MXM_HIP(launch_diagnostics(g.dim, g.P, false, &a, stream));
+      MXM_HIP(launch_diagnostics(g.dim, g.P, false, &a, g.stream));
     }
-    MXM_HIP(hipMemcpyAsync(h, g.dout.p, (3 * NC + 2) * sizeof(double), hipMemcpyDeviceToHost, stream));
-    MXM_HIP(hipStreamSynchronize(stream));
+    MXM_HIP(hipMemcpyAsync(h, g.dout.p, (3 * NC + 2) * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    MXM_HIP(hipStreamSynchronize(g.stream));
     if (!diag_nvel(g.phys)) h[3 * NC] = h[3 * NC + 1] = std::nan("");
   }
 
+  // drops an instance: a waiting one is forgotten; a placed one is
+  // switched off on the device (in stream order) and its tiles become a
+  // free range; its slot is reused only after the next compaction
   void remove(int id) {
     BInst &in = inst(id);
-    (void)in;
     BGroup &g = group_with(id);
-    MXM_HIP(hipStreamSynchronize(stream));
-    pull_counters(g);
-    g.members.erase(std::find(g.members.begin(), g.members.end(), id));
-    g.dirty = true;
+    auto w = std::find(g.waiting.begin(), g.waiting.end(), id);
+    if (w != g.waiting.end()) {
+      g.waiting.erase(w);
+    } else if (in.placed) {
+      hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, g.stream, g.ar.iactive.p + in.slot, 0, 1);
+      const int nt = in.ntiles();
+      hipLaunchKernelGGL(fill_int_kernel, dim3((nt + 255) / 256), dim3(256), 0, g.stream,
+                         g.ar.tile_inst.p + in.base_tile, 0, nt);
+      MXM_HIP(hipGetLastError());
+      g.fs.give(in.base_tile, nt);
+      g.placed.erase(std::find(g.placed.begin(), g.placed.end(), id));
+      if (g.placed.empty() && g.waiting.empty()) release(g);
+    }
     insts.erase(id);
+  }
+
+  // a group with no instances gives its arena back at once
+  void release(BGroup &g) {
+    drain(g);
+    for (auto &e : g.sg) e.reset();
+    Arena empty;
+    g.ar.swap(empty);
+    g.fs = FreeSpace{};
+    g.cap_slots = 0, g.next_slot = 1, g.maxj = 1, g.grid_tiles = 0;
+    g.calls = g.seen = 0;
   }
 
   size_t bytes() const {
@@ -677,7 +1040,7 @@ int mxm_batch_create(int precision, void **out) {
     if (precision != MXM_PREC_FP64 && precision != MXM_PREC_FP32_MFMA) throw ArgError("unknown precision");
     auto b = std::make_unique<Batch>();
     b->prec = precision;
-    MXM_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    MXM_HIP(hipStreamCreateWithFlags(&b->cstream, hipStreamNonBlocking));
     *out = b.release();
     return MXM_OK;
   });
@@ -726,30 +1089,42 @@ int mxm_batch_step(void *h, int nsteps) {
   });
 }
 
+static int poll_into(Batch *b, int lag, int *ids, int cap, int *count) {
+  if (!count || (cap > 0 && !ids)) throw ArgError("null argument");
+  std::vector<int> done = b->poll(lag);
+  if ((int)done.size() > cap) {
+    // report what fits; the rest stays unreported for the next poll
+    for (size_t k = cap; k < done.size(); ++k) b->inst(done[k]).reported = false;
+    done.resize(cap);
+  }
+  for (size_t k = 0; k < done.size(); ++k) ids[k] = done[k];
+  *count = (int)done.size();
+  return MXM_OK;
+}
+
 int mxm_batch_poll(void *h, int *ids, int cap, int *count) {
-  return guarded([&] {
-    Batch *b = as_batch(h);
-    if (!count || (cap > 0 && !ids)) throw ArgError("null argument");
-    b->sync_counters();
-    int k = 0;
-    for (auto &kv : b->insts) {
-      BInst &in = *kv.second;
-      if (!in.reported && in.steps_done >= in.nsteps && k < cap) {
-        ids[k++] = in.id;
-        in.reported = true;
-      }
-    }
-    *count = k;
-    return MXM_OK;
-  });
+  return guarded([&] { return poll_into(as_batch(h), 0, ids, cap, count); });
+}
+
+int mxm_batch_poll_lagged(void *h, int *ids, int cap, int *count) {
+  return guarded([&] { return poll_into(as_batch(h), 1, ids, cap, count); });
 }
 
 int mxm_batch_status(void *h, int id, int *steps_done) {
   return guarded([&] {
     Batch *b = as_batch(h);
     if (!steps_done) throw ArgError("null argument");
-    b->sync_counters();
+    b->sync_all();
     *steps_done = b->inst(id).steps_done;
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_failed(void *h, int id, int *failed) {
+  return guarded([&] {
+    Batch *b = as_batch(h);
+    if (!failed) throw ArgError("null argument");
+    *failed = b->inst(id).failed ? 1 : 0;
     return MXM_OK;
   });
 }
@@ -797,7 +1172,7 @@ int mxm_batch_memory(void *h, long long *bytes) {
 
 int mxm_batch_sync(void *h) {
   return guarded([&] {
-    MXM_HIP(hipStreamSynchronize(as_batch(h)->stream));
+    as_batch(h)->sync_all();
     return MXM_OK;
   });
 }

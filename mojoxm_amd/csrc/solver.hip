@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -21,6 +22,10 @@
 
 namespace mxm {
 namespace {
+
+// MI355X published peaks: HBM3E bandwidth (GB/s) and the dense FP32
+// matrix rate of V_MFMA_F32_* (TFLOP/s)
+constexpr double kHbmPeakGBs = 8000.0, kMfmaF32PeakTFs = 157.3;
 
 struct Solver {
   int dim = 0, P = 0, NP = 0, NFP = 0, NF = 0, NC = 0, phys = 0;
@@ -50,13 +55,17 @@ struct Solver {
   int Kd = 0;  // device element slots, padding included
   int ntiles = 0;
 
-  // HIP graphs of SSPRK3 steps (kernel nodes only), replayed by step():
-  // [0] holds kGraphSteps steps, [1] one step.  A graph bakes in dt, the
-  // limiter setting and the launch configuration; step() rebuilds both when
-  // any of them changed.  A replayed graph costs about 1 us less per kernel
-  // than stream launches of the same kernels.
-  static constexpr int kGraphSteps = 8;
-  StepGraph gexec[2];
+  // HIP graphs of SSPRK3 steps (kernel nodes only), replayed by step(): one
+  // of kGraphSteps steps and one of exactly r steps for each remainder
+  // r = n mod kGraphSteps asked for, so step(n) is at most n / kGraphSteps
+  // + 1 graph launches.  A graph bakes in dt, the limiter setting and the
+  // launch configuration; all are rebuilt when any of them changed.  A
+  // replayed graph costs about 1 us less per kernel than stream launches of
+  // the same kernels, and one launch of many steps keeps the host out of
+  // the way of short runs.
+  static constexpr int kGraphSteps = 32;
+  StepGraph gfull;
+  std::map<int, StepGraph> grem;
   double g_dt = 0.0, g_eps = 0.0;
   bool g_lim = false;
   int g_cfg = -1;
@@ -67,7 +76,9 @@ struct Solver {
   }
 
   void drop_graphs() {
-    for (auto &e : gexec) e.reset();
+    gfull.reset();
+    for (auto &kv : grem) kv.second.reset();
+    grem.clear();
   }
 
   // n SSPRK3 steps as one HIP graph, kept beside its executable
@@ -151,6 +162,9 @@ struct Solver {
     a.jobs = jobs.p, a.jgeo = jgeo.p, a.ipr = ipr.p, a.maxj = maxj, a.ntiles = ntiles;
     a.post_comp = s == 2 ? post_component(phys, prm) : -1;
     a.post_scale = post_factor(phys, prm, dt);
+    // the limiter's cell means come from the stage epilogue
+    a.mean = limiter ? mean.p : nullptr;
+    a.w = w.p;
     return a;
   }
 
@@ -158,7 +172,8 @@ struct Solver {
   void stage_with(int s, double dt, R *b0, R *b1, R *b2) {
     const StageArgs<R> a = stage_args(s, dt, b0, b1, b2);
     MXM_HIP(launch(&a, stream));
-    if (limiter) MXM_HIP(launch_limiter<R>(a.q_out, mean.p, w.p, conn.p, Kd, NP, NC, NF, dim, eps, stream));
+    if (limiter)
+      MXM_HIP(launch_bj_apply<R>(a.q_out, mean.p, conn.p, Kd, NP, NC, NF, dim, eps, stream, tile_count.p, til.epb));
   }
 
   // times every launch configuration on stage 1 (q -> q1; q is not
@@ -203,13 +218,13 @@ struct Solver {
     if (!diag_nvel(phys)) h[3 * NC] = h[3 * NC + 1] = std::nan("");
   }
 
-  // the largest signal speed over the nodes of the current state
-  double max_wave_speed() {
-    if (dpart.n < (size_t)ntiles) dpart.alloc(ntiles);
-    if (dout.n < 1) dout.alloc(1);
+  // enqueues the largest-signal-speed reduction of the current state on
+  // the solver's stream: tile maxima into part[ntiles], the maximum into
+  // out[0] (both device memory)
+  void enqueue_max_speed(double *part, double *out) {
     auto fill = [&](auto &a) {
       a.prm = dprm.p, a.tile_inst = nullptr, a.tile_count = tile_count.p;
-      a.epb = til.epb, a.NP = NP, a.tile0 = 0, a.ntiles = ntiles, a.part = dpart.p, a.out = dout.p;
+      a.epb = til.epb, a.NP = NP, a.tile0 = 0, a.ntiles = ntiles, a.part = part, a.out = out;
     };
     if (prec == MXM_PREC_FP64) {
       SpeedArgs<double> a{};
@@ -222,22 +237,36 @@ struct Solver {
       fill(a);
       MXM_HIP(launch_max_speed(phys, false, &a, stream));
     }
+  }
+
+  // the largest signal speed over the nodes of the current state
+  double max_wave_speed() {
+    if (dpart.n < (size_t)ntiles) dpart.alloc(ntiles);
+    if (dout.n < 1) dout.alloc(1);
+    enqueue_max_speed(dpart.p, dout.p);
     double h = 0.0;
     MXM_HIP(hipMemcpyAsync(&h, dout.p, sizeof(double), hipMemcpyDeviceToHost, stream));
     MXM_HIP(hipStreamSynchronize(stream));
     return h;
   }
 
-  void step(double dt, int n) {
+  // builds (if needed) the graphs that step(dt, n) replays
+  void prepare(double dt, int n) {
     if (n <= 0) return;
     if (dt != g_dt || limiter != g_lim || eps != g_eps || cfg != g_cfg) {
       drop_graphs();
       g_dt = dt, g_lim = limiter, g_eps = eps, g_cfg = cfg;
     }
-    if (n >= kGraphSteps && !gexec[0].exec) capture_steps(dt, kGraphSteps, gexec[0]);
-    if (n % kGraphSteps && !gexec[1].exec) capture_steps(dt, 1, gexec[1]);
-    for (; n >= kGraphSteps; n -= kGraphSteps) MXM_HIP(hipGraphLaunch(gexec[0].exec, stream));
-    for (; n > 0; --n) MXM_HIP(hipGraphLaunch(gexec[1].exec, stream));
+    if (n >= kGraphSteps && !gfull.exec) capture_steps(dt, kGraphSteps, gfull);
+    const int r = n % kGraphSteps;
+    if (r && !grem[r].exec) capture_steps(dt, r, grem[r]);
+  }
+
+  void step(double dt, int n) {
+    if (n <= 0) return;
+    prepare(dt, n);
+    for (; n >= kGraphSteps; n -= kGraphSteps) MXM_HIP(hipGraphLaunch(gfull.exec, stream));
+    if (n) MXM_HIP(hipGraphLaunch(grem[n].exec, stream));
   }
 
   long long dof() const { return (long long)K * NP * NC; }
@@ -536,6 +565,15 @@ int mxm_dg_rk_stage(void *h, int stage, double dt) {
   });
 }
 
+int mxm_dg_prepare_steps(void *h, double dt, int nsteps) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    if (nsteps < 0) throw ArgError("negative step count");
+    s->prepare(dt, nsteps);
+    return MXM_OK;
+  });
+}
+
 int mxm_dg_step_ssprk3(void *h, double dt, int nsteps) {
   return guarded([&] {
     Solver *s = as_solver(h);
@@ -568,10 +606,18 @@ int mxm_dg_bench_step_loop(void *h, double dt, int warmup, int measure, mxm_thro
     if (!out || warmup < 0 || measure <= 0) throw ArgError("bad bench arguments");
     s->step(dt, warmup);
     MXM_HIP(hipStreamSynchronize(s->stream));
+    hipEvent_t ev[2];
+    for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
     auto t0 = std::chrono::steady_clock::now();
+    MXM_HIP(hipEventRecord(ev[0], s->stream));
     s->step(dt, measure);
+    MXM_HIP(hipEventRecord(ev[1], s->stream));
     MXM_HIP(hipStreamSynchronize(s->stream));
     auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.f;
+    const hipError_t erc = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    MXM_HIP(erc);
     const double wall = std::chrono::duration<double>(t1 - t0).count();
     out->steps = measure;
     out->wall_seconds = wall;
@@ -580,6 +626,18 @@ int mxm_dg_bench_step_loop(void *h, double dt, int warmup, int measure, mxm_thro
     out->dof_per_second = (double)s->dof() * measure / wall;
     out->dof_stage_updates_per_second = 3.0 * out->dof_per_second;
     out->state_bandwidth_gbs = (double)s->state_bytes_per_step() * measure / wall * 1e-9;
+    const double st = (double)ms * 1e-3 / (3.0 * measure);
+    out->stage_seconds = st;
+    out->achieved_hbm_gbs = (double)s->state_bytes_per_step() / 3.0 / st * 1e-9;
+    out->hbm_fraction = out->achieved_hbm_gbs / kHbmPeakGBs;
+    if (s->prec == MXM_PREC_FP32_MFMA) {
+      const double flops = 2.0 * (double)s->dof() * (s->dim * s->NP + s->NF * s->NFP);
+      out->mfma_tflops = flops / st * 1e-12;
+      out->mfma_utilization = out->mfma_tflops / kMfmaF32PeakTFs;
+    } else {
+      out->mfma_tflops = 0.0;
+      out->mfma_utilization = 0.0;
+    }
     return MXM_OK;
   });
 }
@@ -629,9 +687,10 @@ int mxm_dg_time_stage_kernel(void *h, double dt, int nsteps, double *avg_us) {
     s->limiter = false;
     // two events on the solver's stream around nsteps SSPRK3 steps, launched
     // as the stepping loop launches them (replayed graphs): the average
-    // includes the launch-to-launch gaps
-    s->step(dt, 1);  // builds the graphs outside the timed region
-    s->step(dt, Solver::kGraphSteps);
+    // includes the launch-to-launch gaps.  The graphs are built, and one
+    // step run, before the timed region.
+    s->prepare(dt, nsteps);
+    s->step(dt, 1);
     hipEvent_t ev[2];
     for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
     MXM_HIP(hipEventRecord(ev[0], s->stream));

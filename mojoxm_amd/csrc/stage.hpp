@@ -63,6 +63,9 @@ struct StageArgs {
   const double *inst_dt;    // [n_inst] or nullptr -> dt
   const double *inst_post;  // [n_inst] or nullptr -> post_scale
   const int *inst_active;   // [n_inst] or nullptr (all active)
+  int *inst_fail;           // [n_inst] or nullptr: set to 1 when a stored value is not finite
+  double *mean;             // [slots][NC] or nullptr: cell means sum_i w_i q_out_i of the stored state
+  const double *w;          // [NP] node weights (with mean)
   double dt, post_scale;
   int post_comp;            // component scaled after the combine (GLM psi), -1 none
   const int *tile_count;  // [ntiles] elements in each tile (slots T*EPB ..)
@@ -311,8 +314,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   }
   FaceItem<R, NC> it;
   if (tid < nitems) load_item_records<NFP>(A, tile, tid, e0, it);
-  R qa_in[NC];  // q_a of the own node, in the state precision until 2b
-  if (tid < EPB * NP) {
+  // q_a of the own node, in the state precision until 2b; when q_a is the
+  // stage input (SSPRK3 stage 1) it is read from the LDS copy of the tile
+  const bool qa_is_in = A.q_a == A.q_in;
+  R qa_in[NC];
+  if (tid < EPB * NP && !qa_is_in) {
     const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) qa_in[c] = g[c];
@@ -414,7 +420,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
       if (phys.source_active()) phys.source(q, src);
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) qa[c] = A.a * double(qa_in[c]) + A.b * double(q[c]) + cdt * double(src[c]);
+    for (int c = 0; c < NC; ++c)
+      qa[c] = A.a * double(qa_is_in ? q[c] : qa_in[c]) + A.b * double(q[c]) + cdt * double(src[c]);
   }
   __syncthreads();
 
@@ -578,8 +585,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     for (int c = 0; c < NC; ++c)
       if (c == A.post_comp) out[c] *= post;
     R *g = A.q_out + (size_t)e0 * NP * NC + (size_t)tid * NC;
+    bool bad = false;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) g[c] = R(out[c]);
+    for (int c = 0; c < NC; ++c) {
+      const R v = R(out[c]);
+      g[c] = v;
+      bad = bad || !__builtin_isfinite(v);
+    }
+    if (bad && A.inst_fail) atomicOr(&A.inst_fail[inst], 1);
+    if (A.mean) {
+      // the stored node, for the cell mean below (each thread rewrites
+      // only the entries it has read)
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sW[tid * NC + c] = R(out[c]);
+    }
+  }
+  // ---- 5. cell means of the stored state (limiter) -----------------------
+  // sum_i w_i q_i per (element, component) in node order, FP64, from the
+  // stored values: what a separate pass over q_out would compute
+  if (A.mean) {
+    __syncthreads();
+    for (int p = tid; p < nE * NC; p += S::BLOCK) {
+      const int el = p / NC, c = p - el * NC;
+      double sm = 0.0;
+#pragma unroll
+      for (int n = 0; n < NP; ++n) sm += A.w[n] * double(sW[(el * NP + n) * NC + c]);
+      A.mean[(size_t)(e0 + el) * NC + c] = sm;
+    }
   }
 }
 

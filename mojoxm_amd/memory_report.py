@@ -37,10 +37,20 @@ class MemoryReport:
 
 @dataclass
 class ThroughputReport:
+    """The reference's four fields, plus device-side figures from HIP events
+    around the measured steps (0 when not measured): the time per stage,
+    the state bytes per stage over it against the 8 TB/s HBM peak, and for
+    the FP32-MFMA form the dense contraction flops per stage over it
+    against the 157.3 TFLOP/s FP32 matrix peak."""
     steps: int
     wall_seconds: float
     dof: int
     state_bytes_per_step: int
+    stage_seconds: float = 0.0
+    achieved_hbm_gbs: float = 0.0
+    hbm_fraction: float = 0.0
+    mfma_tflops: float = 0.0
+    mfma_utilization: float = 0.0
 
     def dof_per_second(self) -> float:
         return self.dof * self.steps / self.wall_seconds
@@ -56,3 +66,8 @@ class ThroughputReport:
               f"({self.wall_seconds / self.steps * 1e3:.4f} ms/step)")
         print(f"  {self.dof_per_second() / 1e9:.3f} GDOF/s, {self.dof_stage_updates_per_second():.4e} "
               f"DOF-stage-updates/s, state >= {self.state_bandwidth_gbs():.1f} GB/s")
+        if self.stage_seconds > 0.0:
+            print(f"  device: {self.stage_seconds * 1e6:.2f} us/stage, {self.achieved_hbm_gbs:.0f} GB/s "
+                  f"({self.hbm_fraction * 100:.1f} % of HBM)"
+                  + (f", MFMA {self.mfma_tflops:.2f} TFLOP/s ({self.mfma_utilization * 100:.1f} % of the "
+                     f"FP32 matrix peak)" if self.mfma_tflops > 0.0 else ""))

@@ -105,6 +105,11 @@ class Solver:
     def step_ssprk3(self, dt: float, nsteps: int = 1) -> None:
         L.check(L.lib().mxm_dg_step_ssprk3(self._h, dt, nsteps), "mxm_dg_step_ssprk3")
 
+    def prepare_steps(self, dt: float, nsteps: int) -> None:
+        """Build the replayed graphs step_ssprk3(dt, nsteps) uses, without
+        stepping (keeps graph construction out of a timed region)."""
+        L.check(L.lib().mxm_dg_prepare_steps(self._h, dt, nsteps), "mxm_dg_prepare_steps")
+
     def enable_cell_limiter(self, enabled: bool = True, venkat_eps: float = 0.1) -> None:
         L.check(L.lib().mxm_dg_enable_limiter(self._h, int(enabled), venkat_eps), "mxm_dg_enable_limiter")
 
@@ -126,7 +131,8 @@ class Solver:
         t = L.Throughput()
         L.check(L.lib().mxm_dg_bench_step_loop(self._h, dt, warmup_steps, measure_steps, C.byref(t)),
                 "mxm_dg_bench_step_loop")
-        return ThroughputReport(t.steps, t.wall_seconds, t.dof, t.state_bytes_per_step)
+        return ThroughputReport(t.steps, t.wall_seconds, t.dof, t.state_bytes_per_step, t.stage_seconds,
+                                t.achieved_hbm_gbs, t.hbm_fraction, t.mfma_tflops, t.mfma_utilization)
 
     def memory_report(self) -> MemoryReport:
         m = L.MemReport()
@@ -173,7 +179,8 @@ class Solver:
 
     def time_stage_kernel(self, dt: float, nsteps: int) -> float:
         """Average stage-kernel duration in microseconds (HIP events on the
-        solver's stream); advances the state by nsteps steps."""
+        solver's stream around nsteps steps, limiter off for them and
+        restored after); advances the state by nsteps + 1 steps."""
         v = C.c_double()
         L.check(L.lib().mxm_dg_time_stage_kernel(self._h, dt, nsteps, C.byref(v)), "mxm_dg_time_stage_kernel")
         return v.value

@@ -126,6 +126,17 @@ def test_bench_step_loop(precision):
         assert t.dof == a.dof_count() and t.state_bytes_per_step == a.state_bytes_per_step()
         assert t.dof_per_second() == pytest.approx(t.dof * 10 / t.wall_seconds, rel=1e-12)
         assert t.dof_stage_updates_per_second() == pytest.approx(3 * t.dof_per_second(), rel=1e-12)
+        # device-side figures: the events sit inside the wall-clock interval
+        assert 0 < 30 * t.stage_seconds <= t.wall_seconds * (1 + 1e-6)
+        assert t.achieved_hbm_gbs == pytest.approx(t.state_bytes_per_step / 3 / t.stage_seconds * 1e-9, rel=1e-12)
+        assert t.hbm_fraction == pytest.approx(t.achieved_hbm_gbs / 8000.0, rel=1e-12)
+        if precision == M.PREC_FP32_MFMA:
+            nd, npn = 2, a.NP  # CASES[0] is 2D P=3: 3 faces of P+1 nodes
+            flops = 2.0 * t.dof * (nd * npn + 3 * 4)
+            assert t.mfma_tflops == pytest.approx(flops / t.stage_seconds * 1e-12, rel=1e-12)
+            assert t.mfma_utilization == pytest.approx(t.mfma_tflops / 157.3, rel=1e-12)
+        else:
+            assert t.mfma_tflops == 0.0 and t.mfma_utilization == 0.0
         t.print()
 
 

@@ -153,3 +153,87 @@ def test_batch_memory_matches_footprints():
         used = b.memory()
     # the group adds its operators and a shared-stride padding of the job tables
     assert total <= used <= total + 64 * 1024
+
+
+def test_batch_failed_instance_is_switched_off():
+    """An instance whose state holds a NaN is flagged by the stage kernel,
+    switched off at the end of its first step (its counter stays at 0) and
+    reported by poll as failed; the other instances of its group run on,
+    each equal to its solve alone."""
+    rng = np.random.default_rng(5)
+    phys = ph.Maxwell2D(c=1.0)
+    bcs = M.BoundaryConditions2D(1, 1, 1, 1)
+    dt, P, ns = 2e-3, 3, 6
+    qs = [random_state(phys, 2 * 6 * 5, 10, rng, 2) for _ in range(3)]
+    bad = qs[1].copy()
+    bad[7, 3, 2] = np.nan
+    with Batch(M.PREC_FP64) as b:
+        ids = [b.add((6, 5), (1.0, 1.0), phys, P, q, dt, ns, bcs=bcs) for q in (qs[0], bad, qs[2])]
+        b.step(1)
+        first = b.poll()
+        assert first == [ids[1]] and b.failed(ids[1])
+        assert b.status(ids[1]) == 0
+        b.step(ns)
+        assert sorted(b.poll()) == sorted([ids[0], ids[2]])
+        assert not b.failed(ids[0]) and not b.failed(ids[2])
+        for i, q in ((ids[0], qs[0]), (ids[2], qs[2])):
+            assert np.array_equal(b.get_state(i), alone((6, 5), (1.0, 1.0), phys, P, q, dt, ns, M.PREC_FP64, bcs=bcs))
+
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_FP32_MFMA])
+def test_batch_reuses_free_ranges_and_compacts(precision):
+    """A removed instance's tiles are taken by a later instance that fits
+    them, with no new device memory (no compaction); one that fits nowhere
+    compacts the group to exactly its instances' footprints.  Every
+    instance equals its solve alone, whichever way it was placed."""
+    rng = np.random.default_rng(9)
+    phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
+    dt, P = 1e-3, 3
+    word = np.float32 if precision == M.PREC_FP32_MFMA else np.float64
+
+    def state(nn):
+        return random_state(phys, 2 * nn[0] * nn[1], 10, rng, 2).astype(word).astype(np.float64)
+
+    sizes = {"a": (12, 10), "b": (6, 6), "c": (8, 5), "d": (6, 5), "e": (16, 16)}
+    q = {k: state(v) for k, v in sizes.items()}
+    steps = {"a": 9, "b": 4, "c": 7, "d": 5, "e": 3}
+    fp = {k: footprint(v, (1.0, 1.0), phys, P, precision) for k, v in sizes.items()}
+    with Batch(precision) as b:
+        ids = {k: b.add(sizes[k], (1.0, 1.0), phys, P, q[k], dt, steps[k]) for k in "abc"}
+        b.step(2)
+        m1 = b.memory()
+        assert fp["a"] + fp["b"] + fp["c"] <= m1 <= fp["a"] + fp["b"] + fp["c"] + 64 * 1024
+        b.remove(ids.pop("b"))
+        ids["d"] = b.add(sizes["d"], (1.0, 1.0), phys, P, q["d"], dt, steps["d"])  # fits b's tiles
+        b.step(3)
+        assert b.memory() == m1
+        ids["e"] = b.add(sizes["e"], (1.0, 1.0), phys, P, q["e"], dt, steps["e"])  # fits nowhere: compaction
+        b.step(2)
+        m2 = b.memory()
+        assert fp["a"] + fp["c"] + fp["d"] + fp["e"] <= m2 <= fp["a"] + fp["c"] + fp["d"] + fp["e"] + 64 * 1024
+        b.step(10)
+        assert set(b.poll()) == set(ids.values())
+        for k, i in ids.items():
+            assert np.array_equal(b.get_state(i), alone(sizes[k], (1.0, 1.0), phys, P, q[k], dt, steps[k],
+                                                        precision)), k
+
+
+def test_batch_lagged_poll_reports_everything():
+    """poll(lagged=True) waits only for the work before the most recent
+    step() call; polling that way after every chunk still reports every
+    instance once, and the results equal the solves alone."""
+    rng = np.random.default_rng(13)
+    phys = ph.ShallowWater2D(hll=True)
+    dt, P = 1e-3, 2
+    steps = [3, 11, 6, 17, 1]
+    qs = [random_state(phys, 2 * 5 * 4, 6, rng, 2) for _ in steps]
+    with Batch(M.PREC_FP64) as b:
+        ids = [b.add((5, 4), (1.0, 1.0), phys, P, q, dt, ns) for q, ns in zip(qs, steps)]
+        seen = []
+        for _ in range(8):
+            b.step(3)
+            seen += b.poll(lagged=True)
+        seen += b.poll()
+        assert sorted(seen) == sorted(ids)
+        for i, q, ns in zip(ids, qs, steps):
+            assert np.array_equal(b.get_state(i), alone((5, 4), (1.0, 1.0), phys, P, q, dt, ns, M.PREC_FP64))
