@@ -243,6 +243,16 @@ int mxm_dg_diagnostics(void *solver, double *out, int cap, int *n);
  * mojo:148-154) and discard the wave speeds its flux hooks return
  * (src/solver.mojo:229,266,268). */
 int mxm_dg_max_wave_speed(void *solver, double *speed);
+/* mxm_dg_max_wave_speed without any allocation (no reference counterpart;
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * reduction is enqueued on the solver's stream and writes the speed to
+ * speed_dev, one double of DEVICE memory owned by the caller; it is there
+ * once the stream reaches it (mxm_dg_sync).  The per-tile partials live in
+ * a fixed library array of 2^18 doubles per device (2 MiB), so meshes of
+ * more than 262144 tiles are rejected with MXM_ERR_ARG.  All calls on one
+ * device share that array: calls that overlap on one device race. */
+int mxm_dg_max_wave_speed_noalloc(void *solver, double *speed_dev);
 /* Launch configurations (no reference counterpart; the reference has one
  * launch shape per kernel).  Each configuration of the stage kernel is an
  * occupancy target (waves per SIMD) that bounds its register allocation;

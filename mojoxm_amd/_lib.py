@@ -78,6 +78,7 @@ EXPORTS = {
     "mxm_dg_launch_config": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_diagnostics": ([C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_max_wave_speed": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "mxm_dg_max_wave_speed_noalloc": ([C.c_void_p, C.c_void_p], C.c_int),
     "mxm_dg_destroy": ([C.c_void_p], C.c_int),
     "mxm_batch_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mxm_batch_autotune": ([C.c_void_p, C.c_int], C.c_int),

@@ -23,11 +23,23 @@
 namespace mxm {
 namespace {
 
+// Scratch of mxm_dg_max_wave_speed_noalloc: the per-tile maxima of the
+// speed reduction, one double per tile, for at most kNoAllocTiles tiles
+// (2 MiB).  Every device holds its own copy for the life of the process;
+// its address on a device comes from hipGetSymbolAddress with that device
+// current (mxm_hip_init), and host code uses only that address
+// (g_scratch), never the array's name.  All calls on one device share it:
+// two calls that overlap on one device race.
+constexpr int kNoAllocTiles = 1 << 18;
+static __device__ __align__(256) double scratch[kNoAllocTiles];
+constexpr int kMaxDevices = 64;
 // MI355X published peaks: HBM3E bandwidth (GB/s) and the dense FP32
 // matrix rate of V_MFMA_F32_* (TFLOP/s)
 constexpr double kHbmPeakGBs = 8000.0, kMfmaF32PeakTFs = 157.3;
+double *g_scratch[kMaxDevices] = {};
 
 struct Solver {
+  int device = 0;  // the device current at create time
   int dim = 0, P = 0, NP = 0, NFP = 0, NF = 0, NC = 0, phys = 0;
   int K = 0, Nf = 0;
   double prm[MXM_PRM_MAX] = {0};
@@ -325,6 +337,7 @@ int create(int dim, const void *desc, int P, int phys, const double *prm_in, int
   if (dim == 3 && phys > MXM_PHYS_EULER && phys < MXM_PHYS_SW) throw ArgError("2D module physics on a 3D mesh");
   if (dim == 2 && phys >= MXM_PHYS_SW) throw ArgError("3D physics on a 2D mesh");
   auto s = std::make_unique<Solver>();
+  s->device = g_device;
   s->phys = phys;
   s->NC = nc;
   s->prec = precision;
@@ -359,7 +372,14 @@ int mxm_hip_init(int device) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(MXM_ERR_NOGPU, "no HIP device");
     if (device < 0 || device >= n) throw ArgError("device index out of range");
+    if (device >= kMaxDevices) throw ArgError("device index beyond " + std::to_string(kMaxDevices));
     MXM_HIP(hipSetDevice(device));
+    if (!g_scratch[device]) {
+      double *p = nullptr;
+      MXM_HIP(hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch)));
+      if (!p) throw HipError("hipGetSymbolAddress(scratch) returned a null address");
+      g_scratch[device] = p;
+    }
     g_device = device;
     return MXM_OK;
   });
@@ -748,6 +768,20 @@ int mxm_dg_max_wave_speed(void *h, double *speed) {
     Solver *s = as_solver(h);
     if (!speed) throw ArgError("null output");
     *speed = s->max_wave_speed();
+    return MXM_OK;
+  });
+}
+
+int mxm_dg_max_wave_speed_noalloc(void *h, double *speed_dev) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    if (!speed_dev) throw ArgError("null output");
+    if (s->ntiles > kNoAllocTiles)
+      throw ArgError("mesh has " + std::to_string(s->ntiles) + " tiles; the no-allocation reduction holds at most " +
+                     std::to_string(kNoAllocTiles));
+    double *part = g_scratch[s->device];
+    if (!part) return fail(MXM_ERR_NOGPU, "mxm_hip_init has not been called for the solver's device");
+    s->enqueue_max_speed(part, speed_dev);
     return MXM_OK;
   });
 }

@@ -158,6 +158,15 @@ class Solver:
         L.check(L.lib().mxm_dg_max_wave_speed(self._h, C.byref(v)), "mxm_dg_max_wave_speed")
         return v.value
 
+    def max_wave_speed_noalloc(self, speed_dev_ptr: int) -> None:
+        """Enqueue the same reduction with no allocation
+        (mxm_dg_max_wave_speed_noalloc): the speed goes to the one double of
+        device memory at speed_dev_ptr once the solver's stream reaches it
+        (synchronize()); the tile partials use the library's fixed scratch
+        array (meshes of at most 262144 tiles)."""
+        L.check(L.lib().mxm_dg_max_wave_speed_noalloc(self._h, C.c_void_p(speed_dev_ptr)),
+                "mxm_dg_max_wave_speed_noalloc")
+
     def autotune(self, dt: float, reps: int = 20):
         """Time every launch configuration of the stage kernel on this mesh
         (stage 1 of SSPRK3, which leaves the state unchanged) and keep the
