@@ -46,7 +46,7 @@ TRAFFIC_PROFILES = {
 
 # occupancy target (waves per SIMD) of each launch configuration; 1 = the
 # compiler's own register budget
-CFG_WAVES = {"fp64": ["compiler", "5"], "fp32_mfma": ["compiler", "5", "6"]}
+CFG_WAVES = {"fp64": ["compiler", "5", "4"], "fp32_mfma": ["compiler", "5", "6", "7"]}
 
 
 def rel_l2(a, b):
@@ -205,6 +205,33 @@ def advection_leg(M, args):
     return out
 
 
+def profile_batch(M, args) -> int:
+    """The batched ensemble for a kernel tracer: the same 512 instances in
+    one batch per arithmetic form, every instance `args.profile_batch`
+    steps, the kernels launched directly on the streams (no graph replay).
+    Prints one JSON line with the steps and the wall time per form."""
+    ens = M.problems.alfven_ensemble(count=args.ensemble)
+    out = {"workload": f"{len(ens)} GLM-MHD Alfven-wave instances (seed 0), P=3, NC=7, n in [16,96]",
+           "steps": args.profile_batch, "launch": "direct stream launches (no graphs)"}
+    states = {}
+    for m in ens:
+        mesh = M.LocalMesh2D(*m.n, *m.L, P=m.P)
+        states[m.index] = m.ic(mesh.elem_node_xyz).astype(np.float32).astype(np.float64)
+    for name, prec in (("fp32_mfma", M.PREC_FP32_MFMA), ("fp64", M.PREC_FP64)):
+        with M.batch.Batch(prec) as b:
+            b.graphs(False)
+            for m in ens:
+                b.add(m.n, m.L, m.physics, m.P, states[m.index], m.dt, args.profile_batch)
+            b.step(0)
+            b.synchronize()
+            t0 = time.perf_counter()
+            b.step(args.profile_batch)
+            b.synchronize()
+            out[name] = {"wall_s": time.perf_counter() - t0, "device_bytes": b.memory()}
+    print(json.dumps(out))
+    return 0
+
+
 def taylor_green_leg(M, args):
     """C3: compressible Taylor-Green vortex, 32^3 cubes x 6 tets, P=3,
     Rusanov, 200 SSPRK3 steps, FP64 against FP32-MFMA from the same
@@ -274,19 +301,61 @@ def serving_leg(M, args):
     # mesh, tables, buffers, upload, autotuning), which the served
     # makespan also contains
     arrivals = sorted(reqs, key=lambda r: r.arrival)
+    out["failed_requests"] = [{"index": r.index, "kind": r.kind, "n": r.n[0], "P": r.P, "steps": r.nsteps}
+                              for r in reqs if r.index in rep.failed]
     for name, prec in (("fp64", M.PREC_FP64), ("fp32_mfma", M.PREC_FP32_MFMA)):
-        setup = {}
-        secs, _ = M.serve.one_at_a_time(reqs, prec, states=states, tune=True, setup=setup)
+        setup, failed = {}, []
+        secs, _ = M.serve.one_at_a_time(reqs, prec, states=states, tune=True, setup=setup, failed=failed)
         both = {r.index: secs[r.index] + setup[r.index] for r in reqs}
-        leg = {}
+        ok = [k for k, r in enumerate(arrivals) if r.index not in failed]
+        leg = {"failed": len(failed)}
         for label, t in (("solve", secs), ("with_setup", both)):
             fifo = M.serve.fifo_latencies(reqs, t)
             makespan = max(r.arrival + l for r, l in zip(arrivals, fifo)) - arrivals[0].arrival
-            leg[label] = {"solves_per_s": len(reqs) / makespan, "p50_s": float(np.percentile(fifo, 50)),
-                          "p99_s": float(np.percentile(fifo, 99))}
+            lat = [fifo[k] for k in ok]
+            leg[label] = {"solves_per_s": len(ok) / makespan, "p50_s": float(np.percentile(lat, 50)),
+                          "p99_s": float(np.percentile(lat, 99))}
         out[f"one_at_a_time_{name}"] = leg
         out[f"speedup_vs_one_at_a_time_{name}"] = rep.solves_per_s / leg["solve"]["solves_per_s"]
     return out
+
+
+def serve_study(M, args) -> int:
+    """The serving stream alone, at each arrival rate of args.rates: the
+    same seeded requests (args.requests of them) re-timed with each rate's
+    Poisson arrivals, served with continuous batching (FP32-MFMA), against
+    FIFO one-at-a-time FP32-MFMA solves, whose solve times do not depend on
+    the arrivals and are measured once.  Prints one JSON line."""
+    base = M.problems.serving_stream(count=args.requests, rate=1.0)
+    states = {r.index: M.serve.initial_state(r).astype(np.float32).astype(np.float64) for r in base}
+    setup, failed = {}, []
+    secs, _ = M.serve.one_at_a_time(base, M.PREC_FP32_MFMA, states=states, tune=True, setup=setup, failed=failed)
+    out = {"workload": f"{len(base)} requests (seed 0): SW dam break, Euler Sod, Maxwell PEC cavity; "
+                       "n in [16,256], P in {2,3,4}, T in [0.1,1]",
+           "precision": "fp32_mfma", "one_at_a_time_failed": sorted(failed),
+           "one_at_a_time_solve_s_total": sum(secs.values()), "one_at_a_time_setup_s_total": sum(setup.values()),
+           "rates": {}}
+    import dataclasses
+    for rate in args.rates:
+        # the same requests, arrivals rescaled to this rate (Poisson at 1/s scaled)
+        reqs = [dataclasses.replace(r, arrival=r.arrival / rate) for r in base]
+        rep = M.serve.serve(reqs, M.PREC_FP32_MFMA, budget=64 << 30, chunk=16, states=states)
+        arrivals = sorted(reqs, key=lambda r: r.arrival)
+        ok = [k for k, r in enumerate(arrivals) if r.index not in failed]
+        leg = {"served": {"solves_per_s": rep.solves_per_s, "solves": rep.solves, "failed": sorted(rep.failed),
+                          "p50_s": rep.pct(50), "p99_s": rep.pct(99), "peak_bytes": rep.peak_bytes,
+                          "makespan_s": rep.makespan}}
+        for label, t in (("solve", secs), ("with_setup", {k: secs[k] + setup[k] for k in secs})):
+            fifo = M.serve.fifo_latencies(reqs, t)
+            makespan = max(r.arrival + l for r, l in zip(arrivals, fifo)) - arrivals[0].arrival
+            lat = [fifo[k] for k in ok]
+            leg[f"one_at_a_time_{label}"] = {"solves_per_s": len(ok) / makespan, "p50_s": float(np.percentile(lat, 50)),
+                                             "p99_s": float(np.percentile(lat, 99)), "makespan_s": makespan}
+        leg["speedup_vs_one_at_a_time"] = rep.solves_per_s / leg["one_at_a_time_solve"]["solves_per_s"]
+        out["rates"][str(rate)] = leg
+        print(json.dumps({"rate_done": rate, **leg}), flush=True)
+    print(json.dumps(out))
+    return 0
 
 
 def main() -> int:
@@ -305,6 +374,13 @@ def main() -> int:
     ap.add_argument("--solo-sample", type=int, default=32, help="ensemble instances also solved one at a time")
     ap.add_argument("--requests", type=int, default=16, help="requests of the serving stream")
     ap.add_argument("--rate", type=float, default=8.0, help="serving arrival rate, requests/s")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving stream (--requests of them) at each rate of --rates")
+    ap.add_argument("--rates", type=lambda v: [float(x) for x in v.split(",")], default=[4.0, 16.0],
+                    help="arrival rates for --serve-only, requests/s, comma separated")
+    ap.add_argument("--profile-batch", type=int, default=0, metavar="STEPS",
+                    help="only run the batched ensemble for STEPS steps per instance, both forms, kernels "
+                         "launched without graphs (for a kernel tracer)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
@@ -316,6 +392,11 @@ def main() -> int:
         return 2
 
     import mojoxm_amd as M
+
+    if args.profile_batch:
+        return profile_batch(M, args)
+    if args.serve_only:
+        return serve_study(M, args)
 
     pb = M.problems.euler_vortex_2d()
     K_el = pb.n[0] * pb.n[1] * 2

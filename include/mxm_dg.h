@@ -257,8 +257,10 @@ int mxm_dg_max_wave_speed_noalloc(void *solver, double *speed_dev);
  * launch shape per kernel).  Each configuration of the stage kernel is an
  * occupancy target (waves per SIMD) that bounds its register allocation;
  * all compute the same bits.  mxm_dg_autotune times each on this solver's
- * mesh -- reps launches of SSPRK3 stage 1, which writes a work buffer and
- * leaves the state unchanged -- and keeps the fastest: *cfg is its index,
+ * mesh -- one SSPRK3 stage 1 into the first work buffer, then reps launches
+ * of stage 2, which reads it and the state and writes the second work
+ * buffer, so the state is left unchanged -- and keeps the fastest: *cfg is
+ * its index,
  * us[0..cap) the microseconds per launch of each configuration
  * (mxm_get_constant(2 / 3) = number of configurations of the FP64 /
  * FP32-MFMA form).  mxm_dg_launch_config sets configuration cfg (cfg < 0:
@@ -284,6 +286,11 @@ int mxm_batch_create(int precision, void **batch);
  * is timed on the group's arena and the fastest kept, once per group and
  * power-of-two range of its tile count. */
 int mxm_batch_autotune(void *batch, int on);
+/* on = 1 (default): each chunk of steps is a replayed HIP graph; on = 0:
+ * the same kernels are launched directly on the group's stream (no
+ * reference counterpart; for tools that cannot follow graph launches, such
+ * as a kernel tracer). */
+int mxm_batch_graphs(void *batch, int on);
 /* Queue an instance: mesh, order, physics (id + prm[MXM_PRM_MAX]), time
  * step dt, step count, limiter on/off with its eps, initial state q0
  * [K*NP*NC] (reference order, FP64).  It joins at the next

@@ -71,6 +71,11 @@ class Batch:
         and tile-count range when groups are packed (on by default)."""
         L.check(L.lib().mxm_batch_autotune(self._h, int(bool(on))), "mxm_batch_autotune")
 
+    def graphs(self, on: bool = True) -> None:
+        """Replay each chunk of steps as a HIP graph (default), or launch
+        the same kernels directly on the streams (for kernel tracers)."""
+        L.check(L.lib().mxm_batch_graphs(self._h, int(bool(on))), "mxm_batch_graphs")
+
     def add(self, n, L_box, physics, P, q0, dt, nsteps, bcs=None, limiter=False, eps=0.1) -> int:
         """Queue one instance; it joins at the next step().  Returns its id."""
         ndim = len(n)

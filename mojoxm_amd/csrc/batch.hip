@@ -315,6 +315,9 @@ struct Batch {
   int next_id = 0;
   // autotuned launch configuration per (group, grid-size octave)
   bool tune = true;
+  // steps as replayed HIP graphs (default) or as direct stream launches of
+  // the same kernels
+  bool graphs = true;
   std::map<std::pair<GroupKey, int>, int> tuned;
 
   ~Batch() {
@@ -696,13 +699,14 @@ struct Batch {
     g.grid_tiles = tiles;
     if (tiles > 0) {
       if (tune) choose_config(g);
-      capture(g);
+      if (graphs) capture(g);
     }
   }
 
   // the launch configuration for the group's grid size: from the cache of
-  // this (group, tile-count octave), else timed on stage 1 (q -> q1, which
-  // the next step rewrites before reading)
+  // this (group, tile-count octave), else timed on stage 2 (q1 and q ->
+  // q2, after one stage 1 q -> q1; the next step rewrites q1 and q2 before
+  // reading them)
   void choose_config(BGroup &g) {
     int oct = 0;
     while ((1 << oct) < g.grid_tiles) ++oct;
@@ -711,10 +715,12 @@ struct Batch {
     if (it == tuned.end()) {
       int c;
       if (g.prec == MXM_PREC_FP64) {
-        const StageArgs<double> a = stage_args(g, g.ar.qd, 0);
+        const StageArgs<double> a1 = stage_args(g, g.ar.qd, 0), a = stage_args(g, g.ar.qd, 1);
+        MXM_HIP(g.launch(&a1, g.stream));
         c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr);
       } else {
-        const StageArgs<float> a = stage_args(g, g.ar.qf, 0);
+        const StageArgs<float> a1 = stage_args(g, g.ar.qf, 0), a = stage_args(g, g.ar.qf, 1);
+        MXM_HIP(g.launch(&a1, g.stream));
         c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr);
       }
       it = tuned.emplace(key, c).first;
@@ -786,12 +792,23 @@ struct Batch {
     capture_steps(g, 1, g.sg[1]);
   }
 
-  // k steps of the group from graph `which` (k = its step count)
+  // k steps of the group from graph `which` (k = its step count), or the
+  // same launches made directly on the group's stream
   void launch_graph(BGroup &g, int which, int k) {
     const int need = live_top(g);
     if (need == 0) return;
-    if (need > g.grid_tiles || need <= g.grid_tiles - g.grid_tiles / 8 || !g.sg[which].exec) recapture(g, need);
-    MXM_HIP(hipGraphLaunch(g.sg[which].exec, g.stream));
+    if (need > g.grid_tiles || need <= g.grid_tiles - g.grid_tiles / 8 || (graphs && !g.sg[which].exec))
+      recapture(g, need);
+    if (graphs) {
+      MXM_HIP(hipGraphLaunch(g.sg[which].exec, g.stream));
+    } else {
+      for (int j = 0; j < k; ++j) {
+        if (g.prec == MXM_PREC_FP64)
+          launch_step(g, g.ar.qd, g.stream);
+        else
+          launch_step(g, g.ar.qf, g.stream);
+      }
+    }
     for (int id : g.placed) {
       BInst &in = inst(id);
       if (in.live()) in.steps_enqueued = std::min(in.nsteps, in.steps_enqueued + k);
@@ -1049,6 +1066,19 @@ int mxm_batch_create(int precision, void **out) {
 int mxm_batch_autotune(void *h, int on) {
   return guarded([&] {
     as_batch(h)->tune = on != 0;
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_graphs(void *h, int on) {
+  return guarded([&] {
+    Batch *b = as_batch(h);
+    b->sync_all();
+    for (auto &kv : b->groups) {
+      kv.second->retire_graphs();
+      kv.second->grid_tiles = 0;  // recaptured (or not) at the next step
+    }
+    b->graphs = on != 0;
     return MXM_OK;
   });
 }

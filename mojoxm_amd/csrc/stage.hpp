@@ -89,22 +89,24 @@ constexpr int kPrmMax = 32;  // MXM_PRM_MAX
 // (waves per SIMD, i.e. 256-thread blocks per CU) that bounds the register
 // allocation; all configurations compute the same bits.  The autotuner
 // times them on the problem and keeps the fastest.  FP64 kernels need
-// 100-150 VGPRs, so only 5 waves is worth trying; the FP32 kernels fit
-// 5 or 6 waves, and the compiler's own choice (1: no bound) can be far off
-// (e.g. 250 VGPRs, 1 wave, for the 2D Euler kernels at P = 1).
+// 100-150 VGPRs: 5 waves (<= 102 VGPRs) or 4 (<= 128, which takes the
+// 131-VGPR GLM-MHD kernel from 3 waves to 4); the FP32 kernels fit 5, 6
+// or 7 waves (<= 72 VGPRs: the 2D trait Euler kernel without spills), and
+// the compiler's own choice (1: no bound) can be far off (e.g. 250 VGPRs,
+// 1 wave, for the 2D Euler kernels at P = 1).
 template <class R>
 struct StageCfg;
 template <>
 struct StageCfg<double> {
-  static constexpr int count = 2, dflt = 0;
-  static constexpr int waves[2] = {1, 5};
+  static constexpr int count = 3, dflt = 0;
+  static constexpr int waves[3] = {1, 5, 4};
 };
 template <>
 struct StageCfg<float> {
-  static constexpr int count = 3, dflt = 1;
-  static constexpr int waves[3] = {1, 5, 6};
+  static constexpr int count = 4, dflt = 1;
+  static constexpr int waves[4] = {1, 5, 6, 7};
 };
-constexpr int kStageCfgMax = 3;
+constexpr int kStageCfgMax = 4;
 
 // Elements per 256-thread block: at most 256/NP for NP nodes per element,
 // rounded down to whole lattice cells (2 triangles, 6 tets) when a cell
@@ -575,7 +577,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   }
   __syncthreads();
 
-  // ---- 3b. RK combine (FP64), limit, store per (element, node) -----------
+  // ---- 4. RK combine (FP64), limit per (element, node), tile store -------
+  // Each node thread writes its finished values over its own entries of the
+  // [el][i][c] result (the tile's device order); after a barrier the block
+  // stores the tile's real elements as one contiguous run, every lane
+  // consecutive, so each cache line is written whole at once (per-node
+  // stores at a stride of NC leave partly written lines in L2 that are
+  // written back more than once when many blocks are resident).
   if (node) {
     double out[NC];
 #pragma unroll
@@ -584,27 +592,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       if (c == A.post_comp) out[c] *= post;
-    R *g = A.q_out + (size_t)e0 * NP * NC + (size_t)tid * NC;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sW[tid * NC + c] = R(out[c]);
+  }
+  __syncthreads();
+  {
+    R *g = A.q_out + (size_t)e0 * NP * NC;
+    const int nv = nE * NP * NC;
     bool bad = false;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const R v = R(out[c]);
-      g[c] = v;
-      bad = bad || !__builtin_isfinite(v);
+    for (int k = 0; k < QPT; ++k) {
+      const int t = k * S::BLOCK + tid;
+      if (t < nv) {
+        const R v = sW[t];
+        g[t] = v;
+        bad = bad || !__builtin_isfinite(v);
+      }
     }
     if (bad && A.inst_fail) atomicOr(&A.inst_fail[inst], 1);
-    if (A.mean) {
-      // the stored node, for the cell mean below (each thread rewrites
-      // only the entries it has read)
-#pragma unroll
-      for (int c = 0; c < NC; ++c) sW[tid * NC + c] = R(out[c]);
-    }
   }
   // ---- 5. cell means of the stored state (limiter) -----------------------
   // sum_i w_i q_i per (element, component) in node order, FP64, from the
   // stored values: what a separate pass over q_out would compute
   if (A.mean) {
-    __syncthreads();
     for (int p = tid; p < nE * NC; p += S::BLOCK) {
       const int el = p / NC, c = p - el * NC;
       double sm = 0.0;

@@ -87,6 +87,8 @@ StageLaunch pick_cfg(int phys, int variant, int cfg) {
   }
   if constexpr (StageCfg<R>::count > 2)
     if (cfg == 2) return F::template pick<P, R, StageCfg<R>::waves[2]>(phys, variant);
+  if constexpr (StageCfg<R>::count > 3)
+    if (cfg == 3) return F::template pick<P, R, StageCfg<R>::waves[3]>(phys, variant);
   return nullptr;
 }
 

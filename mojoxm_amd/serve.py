@@ -133,7 +133,8 @@ def serve(requests, precision=PREC_FP64, budget=64 << 30, chunk=16, keep_results
     return ServeReport(len(ok), makespan, [lat[r.index] for r in ok], peak, budget, out, failed)
 
 
-def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False, tune=False, setup=None):
+def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False, tune=False, setup=None,
+                  failed=None, check_every=512):
     """Each request alone with its own Solver, in arrival order.  Returns
     (solve seconds per request, results): the SSPRK3 stepping only.  Pass a
     dict as `setup` to also get each request's set-up seconds (Solver
@@ -141,7 +142,11 @@ def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False
     autotuning), which a real one-at-a-time server pays too.  A FIFO
     server's time-to-solution follows from the arrivals:
     finish_i = max(arrival_i, finish_{i-1}) + t_i.
-    tune: autotune each solver's launch configuration first."""
+    tune: autotune each solver's launch configuration first.
+    Like the served requests, a request stops when its state is no longer
+    finite: every `check_every` steps the device speed reduction (NaN or
+    infinite for a non-finite state) is read, and the indices of requests
+    stopped that way go to the list `failed`."""
     q0 = states or {r.index: initial_state(r) for r in requests}
     secs, out = {}, {}
     for r in sorted(requests, key=lambda r: r.arrival):
@@ -153,12 +158,23 @@ def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False
             if tune:
                 s.autotune(r.dt, 5)
                 s.set_state(q0[r.index])
-            s.prepare_steps(r.dt, r.nsteps)
+            chunk = min(check_every, r.nsteps) if check_every else r.nsteps
+            s.prepare_steps(r.dt, chunk)
+            if r.nsteps % chunk:
+                s.prepare_steps(r.dt, r.nsteps % chunk)
             s.synchronize()
             t = time.perf_counter()
             if setup is not None:
                 setup[r.index] = t - t_setup
-            s.step_ssprk3(r.dt, r.nsteps)
+            done = 0
+            while done < r.nsteps:
+                k = min(chunk, r.nsteps - done)
+                s.step_ssprk3(r.dt, k)
+                done += k
+                if check_every and not np.isfinite(s.max_wave_speed()):
+                    if failed is not None:
+                        failed.append(r.index)
+                    break
             s.synchronize()
             secs[r.index] = time.perf_counter() - t
             if keep_results:

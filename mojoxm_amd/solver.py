@@ -169,8 +169,9 @@ class Solver:
 
     def autotune(self, dt: float, reps: int = 20):
         """Time every launch configuration of the stage kernel on this mesh
-        (stage 1 of SSPRK3, which leaves the state unchanged) and keep the
-        fastest.  Returns (chosen index, microseconds per configuration)."""
+        (stage 2 of SSPRK3 into a work buffer, after one stage 1, which
+        leaves the state unchanged) and keep the fastest.  Returns (chosen
+        index, microseconds per configuration)."""
         us = (C.c_double * 4)()
         cfg = C.c_int()
         L.check(L.lib().mxm_dg_autotune(self._h, dt, reps, C.byref(cfg), us, 4), "mxm_dg_autotune")

@@ -237,3 +237,24 @@ def test_batch_lagged_poll_reports_everything():
         assert sorted(seen) == sorted(ids)
         for i, q, ns in zip(ids, qs, steps):
             assert np.array_equal(b.get_state(i), alone((5, 4), (1.0, 1.0), phys, P, q, dt, ns, M.PREC_FP64))
+
+
+def test_batch_without_graphs_matches_alone():
+    """Direct stream launches (graphs off, the mode for kernel tracers) run
+    the same kernels: every instance still equals its solve alone."""
+    rng = np.random.default_rng(17)
+    cases = make_cases(rng)[:4]
+    with Batch(M.PREC_FP32_MFMA) as b:
+        b.graphs(False)
+        ids, qs = [], []
+        for n, L, phys, P, dt, ns, bcs, lim in cases:
+            q0 = random_state(phys, int(np.prod(n)) * (6 if len(n) == 3 else 2),
+                              (P + 1) * (P + 2) * (P + 3) // 6 if len(n) == 3 else (P + 1) * (P + 2) // 2, rng,
+                              len(n)).astype(np.float32).astype(np.float64)
+            qs.append(q0)
+            ids.append(b.add(n, L, phys, P, q0, dt, ns, bcs=bcs, limiter=lim))
+        b.step(max(c[5] for c in cases))
+        assert sorted(b.poll()) == sorted(ids)
+        for i, q0, (n, L, phys, P, dt, ns, bcs, lim) in zip(ids, qs, cases):
+            assert np.array_equal(b.get_state(i), alone(n, L, phys, P, q0, dt, ns, M.PREC_FP32_MFMA, bcs=bcs,
+                                                        limiter=lim))
