@@ -329,7 +329,12 @@ def serve_study(M, args) -> int:
     base = M.problems.serving_stream(count=args.requests, rate=1.0)
     states = {r.index: M.serve.initial_state(r).astype(np.float32).astype(np.float64) for r in base}
     setup, failed = {}, []
-    secs, _ = M.serve.one_at_a_time(base, M.PREC_FP32_MFMA, states=states, tune=True, setup=setup, failed=failed)
+    def progress(k, n):
+        if k % 16 == 0 or k == n:
+            print(json.dumps({"one_at_a_time_done": k, "of": n}), flush=True)
+
+    secs, _ = M.serve.one_at_a_time(base, M.PREC_FP32_MFMA, states=states, tune=True, setup=setup, failed=failed,
+                                    progress=progress)
     out = {"workload": f"{len(base)} requests (seed 0): SW dam break, Euler Sod, Maxwell PEC cavity; "
                        "n in [16,256], P in {2,3,4}, T in [0.1,1]",
            "precision": "fp32_mfma", "one_at_a_time_failed": sorted(failed),
@@ -339,7 +344,11 @@ def serve_study(M, args) -> int:
     for rate in args.rates:
         # the same requests, arrivals rescaled to this rate (Poisson at 1/s scaled)
         reqs = [dataclasses.replace(r, arrival=r.arrival / rate) for r in base]
-        rep = M.serve.serve(reqs, M.PREC_FP32_MFMA, budget=64 << 30, chunk=16, states=states)
+        def served(k, n, rate=rate):
+            if k % 16 == 0 or k == n:
+                print(json.dumps({"rate": rate, "served_done": k, "of": n}), flush=True)
+
+        rep = M.serve.serve(reqs, M.PREC_FP32_MFMA, budget=64 << 30, chunk=16, states=states, progress=served)
         arrivals = sorted(reqs, key=lambda r: r.arrival)
         ok = [k for k, r in enumerate(arrivals) if r.index not in failed]
         leg = {"served": {"solves_per_s": rep.solves_per_s, "solves": rep.solves, "failed": sorted(rep.failed),

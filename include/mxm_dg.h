@@ -291,6 +291,12 @@ int mxm_batch_autotune(void *batch, int on);
  * reference counterpart; for tools that cannot follow graph launches, such
  * as a kernel tracer). */
 int mxm_batch_graphs(void *batch, int on);
+/* Extra capacity, as a fraction of its tiles and instance slots, of an
+ * arena rebuilt because a new instance did not fit (its first build is
+ * exact; default 0).  A serving loop that admits work continually sets it
+ * so that later arrivals are placed without another rebuild (no reference
+ * counterpart). */
+int mxm_batch_headroom(void *batch, double fraction);
 /* Queue an instance: mesh, order, physics (id + prm[MXM_PRM_MAX]), time
  * step dt, step count, limiter on/off with its eps, initial state q0
  * [K*NP*NC] (reference order, FP64).  It joins at the next

@@ -83,6 +83,7 @@ EXPORTS = {
     "mxm_batch_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "mxm_batch_autotune": ([C.c_void_p, C.c_int], C.c_int),
     "mxm_batch_graphs": ([C.c_void_p, C.c_int], C.c_int),
+    "mxm_batch_headroom": ([C.c_void_p, C.c_double], C.c_int),
     "mxm_batch_diagnostics": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_batch_max_wave_speed": ([C.c_void_p, C.c_int, C.POINTER(C.c_double)], C.c_int),
     "mxm_batch_add_2d": ([C.c_void_p, C.POINTER(Mesh2DDesc), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,

@@ -76,6 +76,11 @@ class Batch:
         the same kernels directly on the streams (for kernel tracers)."""
         L.check(L.lib().mxm_batch_graphs(self._h, int(bool(on))), "mxm_batch_graphs")
 
+    def headroom(self, fraction: float) -> None:
+        """Spare capacity (a fraction of tiles and slots) for arenas that are
+        rebuilt because a new instance did not fit; 0 (default) is exact."""
+        L.check(L.lib().mxm_batch_headroom(self._h, float(fraction)), "mxm_batch_headroom")
+
     def add(self, n, L_box, physics, P, q0, dt, nsteps, bcs=None, limiter=False, eps=0.1) -> int:
         """Queue one instance; it joins at the next step().  Returns its id."""
         ndim = len(n)

@@ -180,6 +180,7 @@ struct BGroup {
   hipStream_t stream = nullptr;
   Arena ar;
   int cap_slots = 0, maxj = 1;
+  int compactions = 0;          // arenas built so far
   FreeSpace fs;                 // tiles of the arena not held by placed instances
   int next_slot = 1;            // slots [1, next_slot) are taken; slot 0 is the dead slot
   std::vector<int> placed;      // instances in the arena
@@ -318,6 +319,9 @@ struct Batch {
   // steps as replayed HIP graphs (default) or as direct stream launches of
   // the same kernels
   bool graphs = true;
+  // extra capacity of an arena built by a compaction after the first, as a
+  // fraction of its tiles and slots (0: exact)
+  double headroom = 0.0;
   std::map<std::pair<GroupKey, int>, int> tuned;
 
   ~Batch() {
@@ -582,9 +586,14 @@ struct Batch {
       tiles += inst(id).ntiles();
       maxj = std::max(maxj, inst(id).t.fj.maxj);
     }
-    const int ni = (int)order.size(), cap_slots = ni + 1;
+    // a group compacted before has grown: give it headroom (a fraction of
+    // its tiles and slots) so that later instances are placed without one
+    const bool grow = g.compactions > 0 && headroom > 0.0;
+    const int ni = (int)order.size();
+    const int cap_slots = ni + 1 + (grow ? std::max(8, (int)(ni * headroom)) : 0);
+    const int cap_tiles = std::max(1, tiles + (grow ? (int)(tiles * headroom) : 0));
     const int epb = g.epb, dd = g.dim * g.dim, NF = g.NF, NFP = g.NFP;
-    const size_t slots = (size_t)std::max(tiles, 1) * epb, tile_vals = (size_t)epb * g.NP * g.NC;
+    const size_t slots = (size_t)cap_tiles * epb, tile_vals = (size_t)epb * g.NP * g.NC;
     Arena na;
     auto state_trio = [&](auto &nq, auto &oq) {
       using R = typename std::remove_reference<decltype(*nq[0].p)>::type;
@@ -611,15 +620,15 @@ struct Batch {
       state_trio(na.qd, g.ar.qd);
     else
       state_trio(na.qf, g.ar.qf);
-    std::vector<double> ij(slots * dd, 0.0), iv(slots, 0.0), geo((size_t)std::max(tiles, 1) * maxj * 4, 0.0);
-    std::vector<int4> cn(slots * NF), jb((size_t)std::max(tiles, 1) * maxj);
-    std::vector<unsigned short> pr((size_t)std::max(tiles, 1) * maxj * NFP, 0);
-    std::vector<int> tc(std::max(tiles, 1), 0), ti(std::max(tiles, 1), 0);
+    std::vector<double> ij(slots * dd, 0.0), iv(slots, 0.0), geo((size_t)cap_tiles * maxj * 4, 0.0);
+    std::vector<int4> cn(slots * NF), jb((size_t)cap_tiles * maxj);
+    std::vector<unsigned short> pr((size_t)cap_tiles * maxj * NFP, 0);
+    std::vector<int> tc(cap_tiles, 0), ti(cap_tiles, 0);
     std::vector<double> iprm((size_t)cap_slots * MXM_PRM_MAX, 0.0), idt(cap_slots, 0.0), ipost(cap_slots, 1.0);
     std::vector<int> iact(cap_slots, 0), ist(cap_slots, 0), inst_n(cap_slots, 0), ifl(cap_slots, 0);
     for (int s = 0; s < (int)slots; ++s)
       for (int lf = 0; lf < NF; ++lf) cn[(size_t)s * NF + lf] = int4{s, 0, 0, 0};
-    for (int T = 0; T < std::max(tiles, 1); ++T)
+    for (int T = 0; T < cap_tiles; ++T)
       for (int j = 0; j < maxj; ++j) jb[(size_t)T * maxj + j] = int4{-1, T * epb, 0, 0};
     std::vector<int> new_base(ni);
     int bt = 0;
@@ -663,8 +672,10 @@ struct Batch {
     g.grid_tiles = 0;
     g.cap_slots = cap_slots, g.maxj = maxj;
     g.fs = FreeSpace{};
-    g.fs.top = g.fs.cap = tiles;
-    g.next_slot = cap_slots;
+    g.fs.top = tiles;
+    g.fs.cap = cap_tiles;
+    g.next_slot = ni + 1;
+    ++g.compactions;
     for (int k = 0; k < ni; ++k) {
       BInst &in = inst(order[k]);
       in.base_tile = new_base[k], in.slot = k + 1, in.placed = true;
@@ -1066,6 +1077,14 @@ int mxm_batch_create(int precision, void **out) {
 int mxm_batch_autotune(void *h, int on) {
   return guarded([&] {
     as_batch(h)->tune = on != 0;
+    return MXM_OK;
+  });
+}
+
+int mxm_batch_headroom(void *h, double fraction) {
+  return guarded([&] {
+    if (!(fraction >= 0.0) || fraction > 16.0) throw ArgError("headroom must be in [0, 16]");
+    as_batch(h)->headroom = fraction;
     return MXM_OK;
   });
 }

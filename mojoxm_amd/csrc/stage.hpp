@@ -225,6 +225,29 @@ __device__ __forceinline__ void load_item_trace(const StageArgs<R> &A, int e0, F
 
 typedef float mxm_float4 __attribute__((ext_vector_type(4)));
 
+// FP64 -> FP32 with stochastic rounding, for the FP32-MFMA form's stored
+// state.  Round-to-nearest storage loses every update smaller than half a
+// float ulp and biases the rest, so over thousands of steps on a nearly
+// steady background the state drifts from the FP64 solution; rounding up
+// or down at random, with probabilities that make the expected stored
+// value equal x, leaves only an unbiased random walk.  The 29 mantissa
+// bits a float drops are cleared (the float of what is left is exact) and
+// one float ulp is added away from zero when a 32-bit draw is below those
+// bits scaled to 32 bits (lo << 3): probability (dropped bits) / 2^29.
+// The draw is the bit reversal of x's low word xor `salt` (one value per
+// node of an element), so its high bits are x's lowest, noisiest bits: a
+// pure function of x and its node, so results do not depend on the launch
+// configuration or on the other instances of a batch.  Infinities and the
+// hardware's NaN have a zero low word and convert unchanged; any other NaN
+// stays non-finite.
+__device__ __forceinline__ float sr_f32(double x, unsigned salt) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)b;
+  const float t = float(__longlong_as_double((long long)(b & ~0x1FFFFFFFull)));
+  const unsigned up = __builtin_bitreverse32(lo ^ salt) < (lo << 3) ? 1u : 0u;
+  return __uint_as_float(__float_as_uint(t) + up);
+}
+
 // W: occupancy target of the launch configuration, in waves per SIMD
 // (amdgpu_waves_per_eu); 1 leaves the register budget to the compiler.
 template <class Phys, int ND, int P, class R, int W>
@@ -592,8 +615,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       if (c == A.post_comp) out[c] *= post;
+    const unsigned salt = unsigned(tid - (tid / NP) * NP) * 0x85EBCA6Bu;  // node in its element
 #pragma unroll
-    for (int c = 0; c < NC; ++c) sW[tid * NC + c] = R(out[c]);
+    for (int c = 0; c < NC; ++c) {
+      if constexpr (kMFMA)
+        sW[tid * NC + c] = sr_f32(out[c], salt);
+      else
+        sW[tid * NC + c] = R(out[c]);
+    }
   }
   __syncthreads();
   {

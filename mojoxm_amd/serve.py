@@ -56,7 +56,7 @@ def _group(req):
 
 
 def serve(requests, precision=PREC_FP64, budget=64 << 30, chunk=16, keep_results=False,
-          states=None) -> ServeReport:
+          states=None, headroom=0.5, progress=None) -> ServeReport:
     """Run the stream with continuous batching.  `chunk` SSPRK3 steps run
     between admission/retirement points; `states` may hold precomputed
     initial states by request index.
@@ -72,7 +72,11 @@ def serve(requests, precision=PREC_FP64, budget=64 << 30, chunk=16, keep_results
     footprints of requests admitted since the last step), plus the new
     request's footprint, plus the footprint of its whole kernel group
     (a compaction holds the old and the new arena of that group at once)
-    must fit the budget."""
+    must fit the budget.  An arena that has had to grow is rebuilt with
+    `headroom` (a fraction) of spare tiles and slots, so that later
+    arrivals are placed without another rebuild; the spare memory counts
+    against the budget like the rest.  progress(done, n), if given, is
+    called as requests complete or fail."""
     reqs = sorted(requests, key=lambda r: r.arrival)
     fps = {r.index: footprint(r.n, r.L, r.physics, r.P, precision, r.limiter, r.bcs) for r in reqs}
     if max(2 * f for f in fps.values()) > budget:
@@ -84,6 +88,7 @@ def serve(requests, precision=PREC_FP64, budget=64 << 30, chunk=16, keep_results
     peak = 0
     lat, out, failed = {}, {}, []
     with Batch(precision) as b:
+        b.headroom(headroom)
         t0 = time.perf_counter()
         unplaced = 0  # footprints added since the last step
         stepped = False
@@ -128,13 +133,15 @@ def serve(requests, precision=PREC_FP64, budget=64 << 30, chunk=16, keep_results
                         out[r.index] = b.get_state(i)
                 b.remove(i)
                 group_fp[_group(r)] -= fps[r.index]
+                if progress:
+                    progress(len(lat) + len(failed), len(reqs))
         makespan = time.perf_counter() - t0 - reqs[0].arrival
     ok = [r for r in reqs if r.index in lat]
     return ServeReport(len(ok), makespan, [lat[r.index] for r in ok], peak, budget, out, failed)
 
 
 def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False, tune=False, setup=None,
-                  failed=None, check_every=512):
+                  failed=None, check_every=512, progress=None):
     """Each request alone with its own Solver, in arrival order.  Returns
     (solve seconds per request, results): the SSPRK3 stepping only.  Pass a
     dict as `setup` to also get each request's set-up seconds (Solver
@@ -146,7 +153,8 @@ def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False
     Like the served requests, a request stops when its state is no longer
     finite: every `check_every` steps the device speed reduction (NaN or
     infinite for a non-finite state) is read, and the indices of requests
-    stopped that way go to the list `failed`."""
+    stopped that way go to the list `failed`.  progress(k, n), if given,
+    is called after each request."""
     q0 = states or {r.index: initial_state(r) for r in requests}
     secs, out = {}, {}
     for r in sorted(requests, key=lambda r: r.arrival):
@@ -179,6 +187,8 @@ def one_at_a_time(requests, precision=PREC_FP64, states=None, keep_results=False
             secs[r.index] = time.perf_counter() - t
             if keep_results:
                 out[r.index] = s.get_state()
+        if progress:
+            progress(len(secs), len(requests))
     return secs, out
 
 

@@ -258,3 +258,33 @@ def test_batch_without_graphs_matches_alone():
         for i, q0, (n, L, phys, P, dt, ns, bcs, lim) in zip(ids, qs, cases):
             assert np.array_equal(b.get_state(i), alone(n, L, phys, P, q0, dt, ns, M.PREC_FP32_MFMA, bcs=bcs,
                                                         limiter=lim))
+
+
+def test_batch_headroom_places_later_instances_without_a_rebuild():
+    """With headroom, an arena rebuilt because an instance did not fit gets
+    spare tiles and slots: the next small instance is placed in them with no
+    new device memory.  Every instance equals its solve alone."""
+    rng = np.random.default_rng(23)
+    phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
+    dt, P = 1e-3, 2
+    sizes = {"a": (10, 8), "c": (6, 6), "d": (4, 4)}
+    q = {k: random_state(phys, 2 * v[0] * v[1], 6, rng, 2) for k, v in sizes.items()}
+    steps = {"a": 7, "c": 5, "d": 3}
+    fp = {k: footprint(v, (1.0, 1.0), phys, P, M.PREC_FP64) for k, v in sizes.items()}
+    with Batch(M.PREC_FP64) as b:
+        b.headroom(0.5)
+        ids = {"a": b.add(sizes["a"], (1.0, 1.0), phys, P, q["a"], dt, steps["a"])}
+        b.step(1)  # first arena: exact
+        assert b.memory() <= fp["a"] + 64 * 1024
+        ids["c"] = b.add(sizes["c"], (1.0, 1.0), phys, P, q["c"], dt, steps["c"])
+        b.step(1)  # does not fit: rebuilt with headroom
+        m2 = b.memory()
+        assert m2 > fp["a"] + fp["c"] + 64 * 1024
+        ids["d"] = b.add(sizes["d"], (1.0, 1.0), phys, P, q["d"], dt, steps["d"])
+        b.step(1)  # placed in the headroom
+        assert b.memory() == m2
+        b.step(10)
+        assert set(b.poll()) == set(ids.values())
+        for k, i in ids.items():
+            assert np.array_equal(b.get_state(i), alone(sizes[k], (1.0, 1.0), phys, P, q[k], dt, steps[k],
+                                                        M.PREC_FP64)), k
