@@ -42,6 +42,9 @@ TRAFFIC_PROFILES = {
     "fp64": os.path.join(ROOT, "profiles", "dg_stage_kernel_euler_vortex_2d_p3.md"),
     "fp32_mfma": os.path.join(ROOT, "profiles", "dg_stage_kernel_fp32_mfma_euler_vortex_2d_p3.md"),
 }
+# the same kernels on the 512^2 vortex (both forms in one summary, one
+# "traffic_bytes_per_launch_<form>:" line each)
+LARGE_PROFILE = os.path.join(ROOT, "profiles", "dg_stage_kernel_euler_vortex_2d_n512_p3.md")
 
 
 # occupancy target (waves per SIMD) of each launch configuration; 1 = the
@@ -53,11 +56,13 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-def measured_traffic(form):
-    """HBM bytes per stage launch from the committed counter summary, or None."""
+def measured_traffic(form, path=None):
+    """HBM bytes per stage launch from a committed counter summary, or None:
+    the C2 summary of `form`, or the line of `form` in the summary at path."""
+    key = "traffic_bytes_per_launch" if path is None else f"traffic_bytes_per_launch_{form}"
     try:
-        with open(TRAFFIC_PROFILES[form]) as f:
-            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read(), re.M)
+        with open(path or TRAFFIC_PROFILES[form]) as f:
+            m = re.search(rf"^{key}:\s*([0-9.eE+]+)", f.read(), re.M)
     except OSError:
         return None
     return float(m.group(1)) if m else None
@@ -232,6 +237,73 @@ def profile_batch(M, args) -> int:
     return 0
 
 
+def large_vortex_leg(M, args, nsteps=None):
+    """The C2 vortex problem at 512^2 squares (524,288 triangles, P=3, U =
+    26.2 M DOF per stage): the same kernels on a state trio larger than the
+    256 MB Infinity Cache in both forms (629 MB FP64, 315 MB FP32), so the
+    roofline fraction is against HBM.  Each form is autotuned, then its
+    stage kernel timed with HIP events; with nsteps, that many steps are
+    also run back to back (the profiling mode)."""
+    pb = M.problems.euler_vortex_2d(N=512)
+    mesh = M.LocalMesh2D(*pb.n, *pb.L, P=pb.P)
+    q0 = pb.ic(mesh.elem_node_xyz).astype(np.float32).astype(np.float64)
+    out = {"workload": "euler_vortex_2d: 512x512x2 tris, P=3, NC=5, HLLEC, CFL 0.15"}
+    for name, prec, word in (("fp64", M.PREC_FP64, 8), ("fp32_mfma", M.PREC_FP32_MFMA, 4)):
+        with M.Solver(pb.n, pb.L, pb.physics, P=pb.P, precision=prec) as s:
+            U = s.dof_count()
+            s.set_state(q0)
+            cfg, cfg_us = s.autotune(pb.dt, 10)
+            s.set_state(q0)
+            if nsteps:
+                s.step_ssprk3(pb.dt, nsteps)
+                s.synchronize()
+            avg_us = s.time_stage_kernel(pb.dt, 30)
+        alg = 8.0 / 3.0 * U * word
+        gbs = alg / (avg_us * 1e-6) / 1e9
+        out[name] = {"kernel_avg_us": avg_us, "algorithmic_bytes_per_launch": alg, "achieved_gbs": gbs,
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(name, LARGE_PROFILE),
+                     "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg],
+                                       "us_per_config": dict(zip(CFG_WAVES[name], cfg_us))}}
+    out["dof_per_stage"] = U
+    out["speedup_fp32_mfma_vs_fp64"] = out["fp64"]["kernel_avg_us"] / out["fp32_mfma"]["kernel_avg_us"]
+    out["traffic_from"] = os.path.relpath(LARGE_PROFILE, ROOT)
+    return out
+
+
+def profile_limited(M, args) -> int:
+    """A limited run for the kernel tracer: the serving stream's Euler Sod
+    tube (HLLE, outflow in x, walls in y, Barth-Jespersen limiter with
+    Venkatakrishnan eps 0.1) on 192^2 x 2 triangles at P = 3, each form
+    autotuned and then stepped args.profile_limited steps.  Prints one
+    JSON line with the time per step."""
+    from mojoxm_amd.boundary import BoundaryConditions2D
+    pr = M.problems
+    n, P = 192, 3
+    phys = M.physics.Euler(gamma=1.4, flux_type=M.physics.FLUX_HLLE)
+    bcs = BoundaryConditions2D(2, 2, 1, 1)
+    dt = pr.apriori_dt(0.15, 1.0 / n, 2.0 * np.sqrt(1.4), P)
+    mesh = M.LocalMesh2D(n, n, 1.0, 1.0, bcs=bcs, P=P)
+    q0 = pr.sod_ic(mesh.elem_node_xyz, n).astype(np.float32).astype(np.float64)
+    out = {"workload": f"euler_sod_2d: {n}x{n}x2 tris, P={P}, NC=5, HLLE, BJ limiter (eps 0.1)",
+           "steps": args.profile_limited}
+    for name, prec in (("fp64", M.PREC_FP64), ("fp32_mfma", M.PREC_FP32_MFMA)):
+        with M.Solver((n, n), (1.0, 1.0), phys, P=P, bcs=bcs, precision=prec) as s:
+            s.enable_cell_limiter(True, 0.1)
+            s.set_state(q0)
+            cfg, _ = s.autotune(dt, 10)
+            s.set_state(q0)
+            s.prepare_steps(dt, args.profile_limited)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step_ssprk3(dt, args.profile_limited)
+            s.synchronize()
+            out[name] = {"ms_per_step": (time.perf_counter() - t0) / args.profile_limited * 1e3,
+                         "launch_config": CFG_WAVES[name][cfg], "dof_per_stage": s.dof_count(),
+                         "finite": bool(np.isfinite(s.get_state()).all())}
+    print(json.dumps(out))
+    return 0
+
+
 def taylor_green_leg(M, args):
     """C3: compressible Taylor-Green vortex, 32^3 cubes x 6 tets, P=3,
     Rusanov, 200 SSPRK3 steps, FP64 against FP32-MFMA from the same
@@ -330,8 +402,7 @@ def serve_study(M, args) -> int:
     states = {r.index: M.serve.initial_state(r).astype(np.float32).astype(np.float64) for r in base}
     setup, failed = {}, []
     def progress(k, n):
-        if k % 16 == 0 or k == n:
-            print(json.dumps({"one_at_a_time_done": k, "of": n}), flush=True)
+        print(json.dumps({"one_at_a_time_done": k, "of": n}), flush=True)
 
     secs, _ = M.serve.one_at_a_time(base, M.PREC_FP32_MFMA, states=states, tune=True, setup=setup, failed=failed,
                                     progress=progress)
@@ -345,8 +416,7 @@ def serve_study(M, args) -> int:
         # the same requests, arrivals rescaled to this rate (Poisson at 1/s scaled)
         reqs = [dataclasses.replace(r, arrival=r.arrival / rate) for r in base]
         def served(k, n, rate=rate):
-            if k % 16 == 0 or k == n:
-                print(json.dumps({"rate": rate, "served_done": k, "of": n}), flush=True)
+            print(json.dumps({"rate": rate, "served_done": k, "of": n}), flush=True)
 
         rep = M.serve.serve(reqs, M.PREC_FP32_MFMA, budget=64 << 30, chunk=16, states=states, progress=served)
         arrivals = sorted(reqs, key=lambda r: r.arrival)
@@ -379,6 +449,7 @@ def main() -> int:
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream leg")
     ap.add_argument("--no-tgv", action="store_true", help="skip the 3D Taylor-Green leg")
     ap.add_argument("--no-advection", action="store_true", help="skip the 3D advection (C1) leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 512^2 vortex leg")
     ap.add_argument("--ensemble", type=int, default=512, help="instances of the batched ensemble")
     ap.add_argument("--solo-sample", type=int, default=32, help="ensemble instances also solved one at a time")
     ap.add_argument("--requests", type=int, default=16, help="requests of the serving stream")
@@ -390,6 +461,10 @@ def main() -> int:
     ap.add_argument("--profile-batch", type=int, default=0, metavar="STEPS",
                     help="only run the batched ensemble for STEPS steps per instance, both forms, kernels "
                          "launched without graphs (for a kernel tracer)")
+    ap.add_argument("--profile-large", type=int, default=0, metavar="STEPS",
+                    help="only the 512^2 vortex leg, with STEPS more steps per form (for the kernel tracer)")
+    ap.add_argument("--profile-limited", type=int, default=0, metavar="STEPS",
+                    help="only a limited Sod run of STEPS steps per form (for the kernel tracer)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 1 GiB copy used to calibrate the HBM byte counters")
     args = ap.parse_args()
@@ -404,6 +479,11 @@ def main() -> int:
 
     if args.profile_batch:
         return profile_batch(M, args)
+    if args.profile_limited:
+        return profile_limited(M, args)
+    if args.profile_large:
+        print(json.dumps(large_vortex_leg(M, args, nsteps=args.profile_large)))
+        return 0
     if args.serve_only:
         return serve_study(M, args)
 
@@ -497,6 +577,7 @@ def main() -> int:
 
     advection = advection_leg(M, args) if not args.no_advection else None
     tgv = taylor_green_leg(M, args) if not args.no_tgv else None
+    large = large_vortex_leg(M, args) if not args.no_large else None
     batched = batched_leg(M, args) if not args.no_batch else None
     serving = serving_leg(M, args) if not args.no_serve else None
 
@@ -527,6 +608,7 @@ def main() -> int:
         },
         "advection": advection,
         "taylor_green": tgv,
+        "vortex_512": large,
         "batched": batched,
         "serving": serving,
         "roofline": r32["roofline"],

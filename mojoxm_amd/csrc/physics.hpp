@@ -456,8 +456,9 @@ struct Euler2D {
   // primitive state with floors (volume and face kernels use the same)
   MXM_HD void prim(const R *q, R &r, R &u, R &v, R &p) const {
     r = q[0] < rmin_ ? rmin_ : q[0];
-    u = q[1] / r;
-    v = q[2] / r;
+    const R ir = R(1) / r;
+    u = q[1] * ir;
+    v = q[2] * ir;
     R ke = R(0.5) * r * (u * u + v * v);
     p = (gamma - R(1)) * (q[3] - ke);
     if (p < pmin_) p = pmin_;
@@ -479,14 +480,14 @@ struct Euler2D {
       R FL[8], FR[8];
       flux<2>(ql, FL);
       flux<2>(qr, FR);
-      R sl = msqrt(ul * ul + vl * vl) + msqrt(gamma * pl / rl);
-      R sr = msqrt(ur * ur + vr * vr) + msqrt(gamma * pr / rr);
+      R sl = msqrt(ul * ul + vl * vl) + msqrt(gamma * pl * (R(1) / rl));
+      R sr = msqrt(ur * ur + vr * vr) + msqrt(gamma * pr * (R(1) / rr));
       R al = sl > sr ? sl : sr;
       for (int c = 0; c < 4; ++c)
         f[c] = R(0.5) * ((FL[c] + FR[c]) * nx + (FL[4 + c] + FR[4 + c]) * ny) - R(0.5) * al * (qr[c] - ql[c]);
     } else {
       // src/local_mesh_2d_gpu_euler.mojo:466-545
-      R al = msqrt(gamma * pl / rl), ar = msqrt(gamma * pr / rr);
+      R al = msqrt(gamma * pl * (R(1) / rl)), ar = msqrt(gamma * pr * (R(1) / rr));
       R unl = ul * nx + vl * ny, unr = ur * nx + vr * ny;
       R FnL[4] = {ql[1] * nx + ql[2] * ny, (ql[1] * ul + pl) * nx + (ql[1] * vl) * ny,
                   (ql[1] * vl) * nx + (ql[2] * vl + pl) * ny, (ul * (ql[3] + pl)) * nx + (vl * (ql[3] + pl)) * ny};
@@ -547,8 +548,8 @@ struct SW2D {
   MXM_HD static SW2D from(const double *p) { return {R(p[0]), R(p[1]), {R(p[3]), R(p[4]), R(p[5])}}; }
   template <int ND>
   MXM_HD void flux(const R *q, R *F) const {
-    R h = q[0] < hmin ? hmin : q[0];
-    R u = q[1] / h, v = q[2] / h, p = R(0.5) * grav * h * h;
+    R h = q[0] < hmin ? hmin : q[0], ih = R(1) / h;
+    R u = q[1] * ih, v = q[2] * ih, p = R(0.5) * grav * h * h;
     F[0] = q[1], F[1] = q[1] * u + p, F[2] = q[1] * v;
     F[3] = q[2], F[4] = q[2] * u, F[5] = q[2] * v + p;
   }
@@ -559,7 +560,8 @@ struct SW2D {
     flux<2>(ql, FL);
     flux<2>(qr, FR);
     R hl = ql[0] < hmin ? hmin : ql[0], hr = qr[0] < hmin ? hmin : qr[0];
-    R ul = ql[1] / hl, vl = ql[2] / hl, ur = qr[1] / hr, vr = qr[2] / hr;
+    const R ihl = R(1) / hl, ihr = R(1) / hr;  // the reciprocals flux<2> took
+    R ul = ql[1] * ihl, vl = ql[2] * ihl, ur = qr[1] * ihr, vr = qr[2] * ihr;
     if constexpr (HLL == 0) {
       R sl = msqrt(ul * ul + vl * vl) + msqrt(grav * hl), sr = msqrt(ur * ur + vr * vr) + msqrt(grav * hr);
       R al = sl > sr ? sl : sr;
@@ -620,10 +622,10 @@ struct MHDGLM2D {
     state(q, F, sp);
   }
   MXM_HD void state(const R *q, R *F, R &speed) const {
-    R r = q[0] < rmin_ ? rmin_ : q[0];
-    R u = q[1] / r, v = q[2] / r, Bx = q[3], By = q[4], E = q[5], psi = q[6];
+    R r = q[0] < rmin_ ? rmin_ : q[0], ir = R(1) / r;
+    R u = q[1] * ir, v = q[2] * ir, Bx = q[3], By = q[4], E = q[5], psi = q[6];
     R BB = Bx * Bx + By * By;
-    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2]) / r;
+    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2]) * ir;
     R p = (gamma - R(1)) * (E - ke - R(0.5) * BB);
     if (p < pmin_) p = pmin_;
     R ps = p + R(0.5) * BB, ch2 = ch * ch;
@@ -631,18 +633,18 @@ struct MHDGLM2D {
     F[4] = u * By - v * Bx, F[5] = (E + ps) * u - Bx * (u * Bx + v * By), F[6] = ch2 * Bx;
     F[7] = q[2], F[8] = q[2] * u - By * Bx, F[9] = q[2] * v + ps - By * By, F[10] = v * Bx - u * By;
     F[11] = psi, F[12] = (E + ps) * v - By * (u * Bx + v * By), F[13] = ch2 * By;
-    R cs2 = gamma * p / r, ca2 = BB / r, s = cs2 + ca2;
-    R disc = s * s - R(4) * cs2 * (Bx * Bx) / r;
+    R cs2 = gamma * p * ir, ca2 = BB * ir, s = cs2 + ca2;
+    R disc = s * s - R(4) * cs2 * (Bx * Bx) * ir;
     if (disc < R(0)) disc = R(0);
     speed = rmax(msqrt(u * u + v * v) + msqrt(R(0.5) * (s + msqrt(disc))), ch);
   }
   // F.n of one side (the x and y fluxes of state() dotted with n) and its
   // Rusanov speed; 7 values live instead of 14
   MXM_HD void side(const R *q, R nx, R ny, R *Fn, R &speed) const {
-    R r = q[0] < rmin_ ? rmin_ : q[0];
-    R u = q[1] / r, v = q[2] / r, Bx = q[3], By = q[4], E = q[5], psi = q[6];
+    R r = q[0] < rmin_ ? rmin_ : q[0], ir = R(1) / r;
+    R u = q[1] * ir, v = q[2] * ir, Bx = q[3], By = q[4], E = q[5], psi = q[6];
     R BB = Bx * Bx + By * By;
-    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2]) / r;
+    R ke = R(0.5) * (q[1] * q[1] + q[2] * q[2]) * ir;
     R p = (gamma - R(1)) * (E - ke - R(0.5) * BB);
     if (p < pmin_) p = pmin_;
     R ps = p + R(0.5) * BB, ub = u * Bx + v * By;
@@ -653,8 +655,8 @@ struct MHDGLM2D {
     Fn[4] = (u * By - v * Bx) * nx + psi * ny;
     Fn[5] = ((E + ps) * u - Bx * ub) * nx + ((E + ps) * v - By * ub) * ny;
     Fn[6] = ch * ch * (Bx * nx + By * ny);
-    R cs2 = gamma * p / r, ca2 = BB / r, s = cs2 + ca2;
-    R disc = s * s - R(4) * cs2 * (Bx * Bx) / r;
+    R cs2 = gamma * p * ir, ca2 = BB * ir, s = cs2 + ca2;
+    R disc = s * s - R(4) * cs2 * (Bx * Bx) * ir;
     if (disc < R(0)) disc = R(0);
     speed = rmax(msqrt(u * u + v * v) + msqrt(R(0.5) * (s + msqrt(disc))), ch);
   }

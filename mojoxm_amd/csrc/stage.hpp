@@ -429,15 +429,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     R F[ND * NC];
     phys.template flux<ND>(q, F);
     const double *iJ = &sJ[el * ND * ND];
+    if constexpr (kMFMA) {
+      // FP32 form: the mapping in FP32 (invJ rounded once per node), as
+      // the volume product that takes its result is
+      float jf[ND * ND];
 #pragma unroll
-    for (int k = 0; k < ND; ++k)
+      for (int t = 0; t < ND * ND; ++t) jf[t] = float(iJ[t]);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        double v = 0.0;
+      for (int k = 0; k < ND; ++k)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * double(F[d * NC + c]);
-        Fk[k * NC + c] = R(v);
-      }
+        for (int c = 0; c < NC; ++c) {
+          float v = 0.f;
+#pragma unroll
+          for (int d = 0; d < ND; ++d) v += jf[k * ND + d] * F[d * NC + c];
+          Fk[k * NC + c] = v;
+        }
+    } else {
+#pragma unroll
+      for (int k = 0; k < ND; ++k)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          double v = 0.0;
+#pragma unroll
+          for (int d = 0; d < ND; ++d) v += iJ[k * ND + d] * double(F[d * NC + c]);
+          Fk[k * NC + c] = R(v);
+        }
+    }
     R src[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) src[c] = R(0);
