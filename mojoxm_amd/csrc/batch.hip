@@ -94,13 +94,16 @@ using GroupKey = std::tuple<int, int, int, int, int, long long>;  // dim, P, phy
 struct Arena {
   DevBuf<double> qd[3];
   DevBuf<float> qf[3];
-  DevBuf<double> mean, invJ, inv_vol, jgeo, iprm, idt, ipost;
+  // limiter (groups with it): cell means [3][slots*NC] and factors
+  // [3][slots] of the stage buffers, node extrema [slots*NC][2]
+  DevBuf<double> lmean, ltheta, lext;
+  DevBuf<double> invJ, inv_vol, jgeo, iprm, idt, ipost;
   DevBuf<int4> conn, jobs;
   DevBuf<unsigned short> ipr;
   DevBuf<int> tile_count, tile_inst, iactive, isteps, insteps, ifail;
   void swap(Arena &o) noexcept {
     for (int k = 0; k < 3; ++k) qd[k].swap(o.qd[k]), qf[k].swap(o.qf[k]);
-    mean.swap(o.mean), invJ.swap(o.invJ), inv_vol.swap(o.inv_vol), jgeo.swap(o.jgeo), iprm.swap(o.iprm);
+    lmean.swap(o.lmean), ltheta.swap(o.ltheta), lext.swap(o.lext), invJ.swap(o.invJ), inv_vol.swap(o.inv_vol), jgeo.swap(o.jgeo), iprm.swap(o.iprm);
     idt.swap(o.idt), ipost.swap(o.ipost), conn.swap(o.conn), jobs.swap(o.jobs), ipr.swap(o.ipr);
     tile_count.swap(o.tile_count), tile_inst.swap(o.tile_inst), iactive.swap(o.iactive), isteps.swap(o.isteps);
     insteps.swap(o.insteps), ifail.swap(o.ifail);
@@ -108,7 +111,7 @@ struct Arena {
   size_t bytes() const {
     size_t b = 0;
     for (int k = 0; k < 3; ++k) b += qd[k].bytes() + qf[k].bytes();
-    return b + mean.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + iprm.bytes() + idt.bytes() +
+    return b + lmean.bytes() + ltheta.bytes() + lext.bytes() + invJ.bytes() + inv_vol.bytes() + jgeo.bytes() + iprm.bytes() + idt.bytes() +
            ipost.bytes() + conn.bytes() + jobs.bytes() + ipr.bytes() + tile_count.bytes() + tile_inst.bytes() +
            iactive.bytes() + isteps.bytes() + insteps.bytes() + ifail.bytes();
   }
@@ -239,7 +242,7 @@ size_t instance_bytes(const ElemTables &t, int NP, int NC, int NFP, int dim, int
   b += js * (sizeof(int4) + 4 * sizeof(double) + NFP * sizeof(unsigned short));  // jobs, geometry, partner nodes
   b += tiles * 2 * sizeof(int);                                     // tile_count, tile_inst
   b += (MXM_PRM_MAX + 2) * sizeof(double) + 4 * sizeof(int);        // per-instance scalars
-  if (limiter) b += slots * NC * sizeof(double);                    // cell means
+  if (limiter) b += slots * (5 * NC + 3) * sizeof(double);          // means, factors, extrema
   return b;
 }
 
@@ -358,7 +361,7 @@ struct Batch {
     g->limiter = in.limiter, g->eps = in.eps;
     g->key = key;
     g->cfg = stage_cfg_default(prec);
-    g->launch = find_stage(in.dim, in.phys, in.variant, in.P, prec, g->cfg);
+    g->launch = find_stage(in.dim, in.phys, in.variant, in.P, prec, g->cfg, g->limiter);
     if (!g->launch) throw ArgError("no stage kernel for this (physics, flux, P)");
     OpTables o = build_operators(re, in.dim, in.NC, prec);
     if (prec == MXM_PREC_FP64) {
@@ -462,6 +465,21 @@ struct Batch {
     g.retired.clear();
   }
 
+  // writes the limited values of slots [s0, s0 + n) of the group's state q
+  // and clears their pending factors (stream order: after the steps
+  // enqueued so far)
+  void materialize(BGroup &g, int s0, int n) {
+    const size_t S = g.ar.ltheta.n / 3;
+    if (n <= 0 || (size_t)s0 + n > S) return;
+    const size_t vo = (size_t)s0 * g.NP * g.NC;
+    if (g.prec == MXM_PREC_FP64)
+      MXM_HIP(launch_bj_apply<double>(g.ar.qd[0].p + vo, g.ar.ltheta.p + s0, g.ar.lmean.p + (size_t)s0 * g.NC, n,
+                                      g.NP, g.NC, g.stream));
+    else
+      MXM_HIP(launch_bj_apply<float>(g.ar.qf[0].p + vo, g.ar.ltheta.p + s0, g.ar.lmean.p + (size_t)s0 * g.NC, n,
+                                     g.NP, g.NC, g.stream));
+  }
+
   // ---- staging ----------------------------------------------------------
   Staging &stage_buf(size_t bytes) {
     for (auto &s : staging)
@@ -517,6 +535,11 @@ struct Batch {
     }
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, g.stream, g.ar.ifail.p + sl, 0, 1);
     MXM_HIP(hipGetLastError());
+    if (g.limiter) {  // nothing pending on the new state (the range may hold a removed instance's factors)
+      const size_t S = g.ar.ltheta.n / 3;
+      for (int b = 0; b < 3; ++b)
+        MXM_HIP(launch_fill(g.ar.ltheta.p + b * S + bs, (long long)in.ntiles() * g.epb, 1.0, g.stream));
+    }
     MXM_HIP(hipEventRecord(st.ev, g.stream));
     st.busy = true;
   }
@@ -570,6 +593,10 @@ struct Batch {
   // (its waiting instances still waiting).
   void compact(BGroup &g) {
     drain(g);
+    if (g.limiter && g.ar.ltheta.n) {  // the states move with their limiter factors applied
+      materialize(g, 0, (int)(g.ar.ltheta.n / 3));
+      drain(g);
+    }
     pull_counters(g);
     // every placed instance stays (finished ones remain readable until
     // they are removed), longest remaining run first: finished and failed
@@ -665,7 +692,12 @@ struct Batch {
     na.isteps.upload(ist.data(), ist.size());
     na.insteps.upload(inst_n.data(), inst_n.size());
     na.ifail.upload(ifl.data(), ifl.size());
-    if (g.limiter) na.mean.alloc(slots * g.NC);
+    if (g.limiter) {
+      na.lmean.alloc(3 * slots * g.NC);
+      na.lext.alloc(2 * slots * g.NC);
+      na.ltheta.alloc(3 * slots);
+      MXM_HIP(launch_fill(na.ltheta.p, (long long)na.ltheta.n, 1.0, g.stream));
+    }
     // ---- commit (no more allocations or copies that can fail) ----
     g.ar.swap(na);
     for (auto &e : g.sg) e.reset();
@@ -728,16 +760,16 @@ struct Batch {
       if (g.prec == MXM_PREC_FP64) {
         const StageArgs<double> a1 = stage_args(g, g.ar.qd, 0), a = stage_args(g, g.ar.qd, 1);
         MXM_HIP(g.launch(&a1, g.stream));
-        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr);
+        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr, g.limiter);
       } else {
         const StageArgs<float> a1 = stage_args(g, g.ar.qf, 0), a = stage_args(g, g.ar.qf, 1);
         MXM_HIP(g.launch(&a1, g.stream));
-        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr);
+        c = autotune_stage(g.dim, g.phys, g.variant, g.P, g.prec, &a, g.stream, 10, nullptr, g.limiter);
       }
       it = tuned.emplace(key, c).first;
     }
     g.cfg = it->second;
-    g.launch = find_stage(g.dim, g.phys, g.variant, g.P, g.prec, g.cfg);
+    g.launch = find_stage(g.dim, g.phys, g.variant, g.P, g.prec, g.cfg, g.limiter);
   }
 
   template <class R>
@@ -750,7 +782,12 @@ struct Batch {
     a.tile_inst = ar.tile_inst.p, a.inst_prm = ar.iprm.p, a.inst_dt = ar.idt.p, a.inst_post = ar.ipost.p;
     a.inst_active = ar.iactive.p;
     a.inst_fail = ar.ifail.p;
-    a.mean = g.limiter ? ar.mean.p : nullptr;
+    if (g.limiter) {  // as in the solver: outputs' means and extrema, inputs' pending factors
+      const size_t S = ar.ltheta.n / 3, kn = S * g.NC;
+      a.mean = ar.lmean.p + pl.out * kn, a.ext = ar.lext.p;
+      a.lim_th_in = ar.ltheta.p + pl.in * S, a.lim_mn_in = ar.lmean.p + pl.in * kn;
+      a.lim_th_a = ar.ltheta.p + pl.qa * S, a.lim_mn_a = ar.lmean.p + pl.qa * kn;
+    }
     a.w = g.w.p;
     a.dt = 0.0, a.post_scale = 1.0;
     // GLM decay per instance (factor 1 where alpha_d = 0)
@@ -768,8 +805,9 @@ struct Batch {
       const StageArgs<R> a = stage_args(g, q, st);
       MXM_HIP(g.launch(&a, s));
       if (g.limiter)
-        MXM_HIP(launch_bj_apply<R>(a.q_out, g.ar.mean.p, g.ar.conn.p, g.grid_tiles * g.epb, g.NP, g.NC, g.NF, g.dim,
-                                   g.eps, s, g.ar.tile_count.p, g.epb, g.ar.tile_inst.p, g.ar.iactive.p));
+        MXM_HIP(launch_bj_theta(g.ar.ltheta.p + (g.ar.ltheta.n / 3) * stage_plan(st).out, a.mean, g.ar.lext.p,
+                                g.ar.conn.p, g.grid_tiles * g.epb, g.NC, g.NF, g.dim, g.eps, s, g.ar.tile_count.p,
+                                g.epb, g.ar.tile_inst.p, g.ar.iactive.p));
     }
     // over every slot of the arena (later placements need no recapture;
     // slots not in use are inactive)
@@ -912,6 +950,8 @@ struct Batch {
   BGroup &settled(int id) {
     BGroup &g = group_with(id);
     if (!inst(id).placed) place(g);
+    const BInst &in = inst(id);
+    if (g.limiter && in.placed) materialize(g, in.base_tile * g.epb, in.ntiles() * g.epb);
     drain(g);
     return g;
   }

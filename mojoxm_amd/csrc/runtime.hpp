@@ -117,16 +117,18 @@ inline int phys_variant(int phys, const double *prm) {
 inline int stage_cfg_count(int prec) { return prec == MXM_PREC_FP64 ? StageCfg<double>::count : StageCfg<float>::count; }
 inline int stage_cfg_default(int prec) { return prec == MXM_PREC_FP64 ? StageCfg<double>::dflt : StageCfg<float>::dflt; }
 
-inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec, int cfg) {
+// lim: the kernel variant for a limited run (limiter epilogue and the
+// inputs' pending factors applied on load)
+inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec, int cfg, bool lim = false) {
   const bool f64 = prec == MXM_PREC_FP64;
   if (cfg < 0 || cfg >= stage_cfg_count(prec)) return nullptr;
   if (dim == 3) {
-    if (phys <= MXM_PHYS_EULER) return (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg);
-    return phys >= MXM_PHYS_SW ? (f64 ? stage_3d_sys_fp64 : stage_3d_sys_fp32)(phys, variant, P, cfg) : nullptr;
+    if (phys <= MXM_PHYS_EULER) return (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg, lim);
+    return phys >= MXM_PHYS_SW ? (f64 ? stage_3d_sys_fp64 : stage_3d_sys_fp32)(phys, variant, P, cfg, lim) : nullptr;
   }
   if (phys >= MXM_PHYS_SW) return nullptr;
-  if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P, cfg);
-  return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P, cfg);
+  if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P, cfg, lim);
+  return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P, cfg, lim);
 }
 
 // Times `launch` over `reps` back-to-back launches (after one untimed
@@ -150,11 +152,11 @@ inline double time_launches(StageLaunch launch, const void *args, hipStream_t st
 // of SSPRK3, which writes q1 from q) and returns the fastest; us[k] gets
 // configuration k's time when us is not null.
 inline int autotune_stage(int dim, int phys, int variant, int P, int prec, const void *args, hipStream_t stream,
-                          int reps, double *us) {
+                          int reps, double *us, bool lim = false) {
   int best = stage_cfg_default(prec);
   double tbest = 0.0;
   for (int c = 0; c < stage_cfg_count(prec); ++c) {
-    StageLaunch l = find_stage(dim, phys, variant, P, prec, c);
+    StageLaunch l = find_stage(dim, phys, variant, P, prec, c, lim);
     const double t = time_launches(l, args, stream, reps);
     if (us) us[c] = t;
     if (c == 0 || t < tbest) best = c, tbest = t;

@@ -52,7 +52,9 @@ struct Solver {
   // state trio q, q1, q2: FP64 (MXM_PREC_FP64) or FP32 (MXM_PREC_FP32_MFMA)
   DevBuf<double> q, q1, q2;
   DevBuf<float> f, f1, f2;
-  DevBuf<double> mean;
+  // limiter: cell means [3][Kd*NC] and factors [3][Kd] of the stage
+  // buffers (q, q1, q2), node extrema [Kd*NC][2] of the last stage output
+  DevBuf<double> lmean, ltheta, lext;
   DevBuf<double> invJ, inv_vol, jgeo, Dt, Lt, w, dprm;
   DevBuf<double> Dn, dpart, dout;  // diagnostics: nodal derivative, tile partials, results
   DevBuf<float> Aop;
@@ -128,7 +130,10 @@ struct Solver {
         MXM_HIP(hipMemset(b->p, 0, b->bytes()));
       }
     }
-    mean.alloc((size_t)Kd * NC);
+    lmean.alloc((size_t)3 * Kd * NC);
+    lext.alloc((size_t)2 * Kd * NC);
+    ltheta.alloc((size_t)3 * Kd);
+    MXM_HIP(launch_fill(ltheta.p, (long long)ltheta.n, 1.0, stream));
     invJ.upload(t.invJ.data(), t.invJ.size());
     inv_vol.upload(t.inv_vol.data(), t.inv_vol.size());
     conn.upload(t.conn.data(), t.conn.size());
@@ -174,8 +179,15 @@ struct Solver {
     a.jobs = jobs.p, a.jgeo = jgeo.p, a.ipr = ipr.p, a.maxj = maxj, a.ntiles = ntiles;
     a.post_comp = s == 2 ? post_component(phys, prm) : -1;
     a.post_scale = post_factor(phys, prm, dt);
-    // the limiter's cell means come from the stage epilogue
-    a.mean = limiter ? mean.p : nullptr;
+    // limiter: the stage epilogue writes the output's cell means and node
+    // extrema, the theta kernel its factors; the inputs' pending factors
+    // are applied as they are read
+    if (limiter) {
+      const size_t kn = (size_t)Kd * NC;
+      a.mean = lmean.p + pl.out * kn, a.ext = lext.p;
+      a.lim_th_in = ltheta.p + (size_t)pl.in * Kd, a.lim_mn_in = lmean.p + pl.in * kn;
+      a.lim_th_a = ltheta.p + (size_t)pl.qa * Kd, a.lim_mn_a = lmean.p + pl.qa * kn;
+    }
     a.w = w.p;
     return a;
   }
@@ -183,9 +195,27 @@ struct Solver {
   template <class R>
   void stage_with(int s, double dt, R *b0, R *b1, R *b2) {
     const StageArgs<R> a = stage_args(s, dt, b0, b1, b2);
-    MXM_HIP(launch(&a, stream));
+    MXM_HIP(kernel()(&a, stream));
     if (limiter)
-      MXM_HIP(launch_bj_apply<R>(a.q_out, mean.p, conn.p, Kd, NP, NC, NF, dim, eps, stream, tile_count.p, til.epb));
+      MXM_HIP(launch_bj_theta(ltheta.p + (size_t)stage_plan(s).out * Kd, a.mean, lext.p, conn.p, Kd, NC, NF, dim,
+                              eps, stream, tile_count.p, til.epb));
+  }
+
+  // the stage kernel of the launch configuration, in its limited variant
+  // when the limiter is on
+  StageLaunch kernel() const { return limiter ? find_stage(dim, phys, variant, P, prec, cfg, true) : launch; }
+
+  // writes the limited values of stage buffer b (0 = q) and clears its
+  // pending factors
+  void materialize(int b) {
+    const size_t kn = (size_t)Kd * NC;
+    if (prec == MXM_PREC_FP64) {
+      double *t[3] = {q.p, q1.p, q2.p};
+      MXM_HIP(launch_bj_apply<double>(t[b], ltheta.p + (size_t)b * Kd, lmean.p + b * kn, Kd, NP, NC, stream));
+    } else {
+      float *t[3] = {f.p, f1.p, f2.p};
+      MXM_HIP(launch_bj_apply<float>(t[b], ltheta.p + (size_t)b * Kd, lmean.p + b * kn, Kd, NP, NC, stream));
+    }
   }
 
   // times every launch configuration on SSPRK3 stage 2 (q1 and q -> q2,
@@ -197,11 +227,11 @@ struct Solver {
     if (prec == MXM_PREC_FP64) {
       stage_with(0, dt, q.p, q1.p, q2.p);
       const StageArgs<double> a = stage_args(1, dt, q.p, q1.p, q2.p);
-      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us);
+      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us, limiter);
     } else {
       stage_with(0, dt, f.p, f1.p, f2.p);
       const StageArgs<float> a = stage_args(1, dt, f.p, f1.p, f2.p);
-      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us);
+      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us, limiter);
     }
     launch = find_stage(dim, phys, variant, P, prec, cfg);
     return cfg;
@@ -283,6 +313,7 @@ struct Solver {
     prepare(dt, n);
     for (; n >= kGraphSteps; n -= kGraphSteps) MXM_HIP(hipGraphLaunch(gfull.exec, stream));
     if (n) MXM_HIP(hipGraphLaunch(grem[n].exec, stream));
+    if (limiter) materialize(0);
   }
 
   long long dof() const { return (long long)K * NP * NC; }
@@ -301,6 +332,7 @@ struct Solver {
         for (size_t k = 0; k < node; ++k) d[(size_t)til.dev_of[e] * node + k] = float(h[(size_t)e * node + k]);
       MXM_HIP(hipMemcpy(f.p, d.data(), n * sizeof(float), hipMemcpyHostToDevice));
     }
+    MXM_HIP(launch_fill(ltheta.p, (long long)ltheta.n, 1.0, stream));  // nothing pending on a new state
   }
   void download_state(double *h) {
     const size_t node = (size_t)NP * NC, n = (size_t)Kd * node;
@@ -585,6 +617,7 @@ int mxm_dg_rk_stage(void *h, int stage, double dt) {
     Solver *s = as_solver(h);
     if (stage < 0 || stage > 2) throw ArgError("stage must be 0, 1 or 2");
     s->stage(stage, dt);
+    if (s->limiter) s->materialize(stage_plan(stage).out);
     return MXM_OK;
   });
 }
@@ -688,7 +721,7 @@ int mxm_dg_memory_report(void *h, mxm_memory_report *o) {
     if (!o) throw ArgError("null out");
     o->rk_stage = (long long)s->state_bytes();
     o->dg_operators = (long long)(s->Dt.bytes() + s->Lt.bytes() + s->Aop.bytes() + s->w.bytes() + s->fmap.bytes());
-    o->limiter = (long long)s->mean.bytes();
+    o->limiter = (long long)(s->lmean.bytes() + s->ltheta.bytes() + s->lext.bytes());
     o->mesh_connectivity = (long long)(s->invJ.bytes() + s->inv_vol.bytes() + s->conn.bytes() +
                                        s->jgeo.bytes() + s->jobs.bytes() + s->ipr.bytes() +
                                        s->tile_count.bytes());

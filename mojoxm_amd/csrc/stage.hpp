@@ -32,8 +32,10 @@
 //               LIFT contractions are one [D | Lift] x [F ; S] product per
 //               tile on V_MFMA_F32_16X16X4F32 with FP32 accumulation, the
 //               surface fluxes pre-scaled by -inv_vol s area; geometry and
-//               Jacobians stay FP64 (the nodal flux is mapped by invJ in
-//               FP64), and so do the RK combine and the positivity rescue.
+//               Jacobians are stored in FP64 (the nodal flux is mapped by
+//               invJ rounded to FP32), and the RK combine and the
+//               positivity rescue are FP64; the stored state is rounded
+//               stochastically.
 // Blocks are mapped to tiles so that each of the 8 XCDs walks a contiguous
 // range of tiles: face neighbours then mostly hit the XCD's own L2.
 #pragma once
@@ -65,7 +67,14 @@ struct StageArgs {
   const int *inst_active;   // [n_inst] or nullptr (all active)
   int *inst_fail;           // [n_inst] or nullptr: set to 1 when a stored value is not finite
   double *mean;             // [slots][NC] or nullptr: cell means sum_i w_i q_out_i of the stored state
+  double *ext;              // [slots][NC][2] (with mean): smallest and largest stored node value
   const double *w;          // [NP] node weights (with mean)
+  // Limiter factors pending on the input buffers (nullptr: none): every
+  // value read from q_in (tile and partner traces) or q_a is used as
+  // mean + theta (q - mean) where its element's theta < 1, which is the
+  // state the limiter would have stored.
+  const double *lim_th_in, *lim_mn_in;  // q_in: theta [slots], means [slots][NC]
+  const double *lim_th_a, *lim_mn_a;    // q_a
   double dt, post_scale;
   int post_comp;            // component scaled after the combine (GLM psi), -1 none
   const int *tile_count;  // [ntiles] elements in each tile (slots T*EPB ..)
@@ -187,6 +196,14 @@ struct FaceItem {
   R qp[NC];  // partner trace when the partner is outside the tile
 };
 
+// A value of the state with its element's pending limiter factor applied:
+// the expression of the limiter's own apply, so the bits are those the
+// limiter would have stored.
+template <class R>
+__device__ __forceinline__ R lim_apply(R v, double th, double bm) {
+  return th < 1.0 ? R(bm + th * (double(v) - bm)) : v;
+}
+
 // Every record of face item t of a tile is addressed from the tile index,
 // so all of them are in flight together (load_item_records); the partner
 // trace of a face to another tile is the only load that waits for one of
@@ -213,13 +230,19 @@ __device__ __forceinline__ void load_item_records(const StageArgs<R> &A, int til
 
 // A partner slot in [0, EPB) is in the tile: partners are real elements,
 // which occupy the tile's first slots.
-template <int NP, int EPB, class R, int NC>
+template <int NP, int EPB, bool LIM, class R, int NC>
 __device__ __forceinline__ void load_item_trace(const StageArgs<R> &A, int e0, FaceItem<R, NC> &it) {
   const bool need = it.packed >= 0 && !(it.side == 0 && it.bc != 0);
   if (need && !(it.pl >= 0 && it.pl < EPB)) {
-    const R *g = A.q_in + ((size_t)(e0 + it.pl) * NP + it.pn) * NC;
+    const size_t p = (size_t)(e0 + it.pl);
+    const R *g = A.q_in + (p * NP + it.pn) * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) it.qp[c] = g[c];
+    if constexpr (LIM) {
+      const double th = A.lim_th_in[p];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) it.qp[c] = lim_apply(it.qp[c], th, A.lim_mn_in[p * NC + c]);
+    }
   }
 }
 
@@ -250,7 +273,7 @@ __device__ __forceinline__ float sr_f32(double x, unsigned salt) {
 
 // W: occupancy target of the launch configuration, in waves per SIMD
 // (amdgpu_waves_per_eu); 1 leaves the register budget to the compiler.
-template <class Phys, int ND, int P, class R, int W>
+template <class Phys, int ND, int P, class R, int W, bool LIM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void dg_stage_kernel(StageArgs<R> A) {
   using S = Shape<ND, P, Phys::NC>;
   constexpr bool kMFMA = std::is_same<R, float>::value;
@@ -290,6 +313,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   __shared__ float sA[kStageA ? MT * 16 * KAS : 1];
   __shared__ double sJ[EPB * ND * ND + EPB];  // invJ then inv_vol
   __shared__ int sMap[NF * NFP];
+  // pending limiter factors of the tile's elements, then their means (q_in)
+  constexpr int NL = EPB * (NC + 1);
+  __shared__ double sLim[LIM ? NL : 1];
   R *const sW = sB;
   R *const sS = sB + TILE;
 
@@ -337,6 +363,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     for (int k = 0; k < APT; ++k)
       if (k * S::BLOCK + tid < NA) ta[k] = A.Aop[k * S::BLOCK + tid];
   }
+  constexpr int LPT = (NL + S::BLOCK - 1) / S::BLOCK;
+  double tl[LIM ? LPT : 1];
+  if constexpr (LIM) {
+#pragma unroll
+    for (int k = 0; k < LPT; ++k) {
+      const int t = k * S::BLOCK + tid;
+      if (t < EPB)
+        tl[k] = A.lim_th_in[e0 + t];
+      else if (t < NL)
+        tl[k] = A.lim_mn_in[(size_t)e0 * NC + t - EPB];
+    }
+  }
   FaceItem<R, NC> it;
   if (tid < nitems) load_item_records<NFP>(A, tile, tid, e0, it);
   // q_a of the own node, in the state precision until 2b; when q_a is the
@@ -351,7 +389,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   const int nE = A.tile_count[tile];
   // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
   const bool node = tid < nE * NP;
-  if (tid < nitems) load_item_trace<NP, EPB>(A, e0, it);
+  if (LIM && !qa_is_in && tid < EPB * NP) {
+    const size_t e = (size_t)e0 + tid / NP;
+    const double th = A.lim_th_a[e];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) qa_in[c] = lim_apply(qa_in[c], th, A.lim_mn_a[e * NC + c]);
+  }
+  if (tid < nitems) load_item_trace<NP, EPB, LIM>(A, e0, it);
 #pragma unroll
   for (int k = 0; k < QPT; ++k)
     if (k * S::BLOCK + tid < TILE) sW[k * S::BLOCK + tid] = tq[k];
@@ -370,14 +414,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   }
   if constexpr (kMFMA)
     for (int t = TILE + NFF * NCOLS + tid; t < NB; t += S::BLOCK) sB[t] = R(0);
+  if constexpr (LIM) {
+#pragma unroll
+    for (int k = 0; k < LPT; ++k)
+      if (k * S::BLOCK + tid < NL) sLim[k * S::BLOCK + tid] = tl[k];
+  }
   __syncthreads();
+  // the q_in tile's pending limiter factors, applied in place to the
+  // values this thread stored
+  if constexpr (LIM) {
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+      const int t = k * S::BLOCK + tid;
+      if (t < TILE) {
+        const int el = t / (NP * NC), c = t % NC;
+        sW[t] = lim_apply(tq[k], sLim[el], sLim[EPB + el * NC + c]);
+      }
+    }
+    __syncthreads();
+  }
   const R *sQ = sW;  // the q_in tile, [el][j][c], until the flux planes are written
 
   // ---- 2a. face fluxes ----------------------------------------------------
   for (int t = tid; t < nitems; t += S::BLOCK) {
     if (t != tid) {
       load_item_records<NFP>(A, tile, t, e0, it);
-      load_item_trace<NP, EPB>(A, e0, it);
+      load_item_trace<NP, EPB, LIM>(A, e0, it);
     }
     if (it.packed < 0) continue;
     const int el = it.packed & 127, lf = (it.packed >> 7) & 3, pin = (it.packed >> 9) & 127;
@@ -660,24 +722,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   // ---- 5. cell means of the stored state (limiter) -----------------------
   // sum_i w_i q_i per (element, component) in node order, FP64, from the
   // stored values: what a separate pass over q_out would compute
-  if (A.mean) {
+  if constexpr (LIM) {
     for (int p = tid; p < nE * NC; p += S::BLOCK) {
       const int el = p / NC, c = p - el * NC;
       double sm = 0.0;
 #pragma unroll
       for (int n = 0; n < NP; ++n) sm += A.w[n] * double(sW[(el * NP + n) * NC + c]);
       A.mean[(size_t)(e0 + el) * NC + c] = sm;
+      {
+        double lo = double(sW[el * NP * NC + c]), hi = lo;
+#pragma unroll
+        for (int n = 1; n < NP; ++n) {
+          const double v = double(sW[(el * NP + n) * NC + c]);
+          lo = v < lo ? v : lo;
+          hi = v > hi ? v : hi;
+        }
+        A.ext[((size_t)(e0 + el) * NC + c) * 2] = lo;
+        A.ext[((size_t)(e0 + el) * NC + c) * 2 + 1] = hi;
+      }
     }
   }
 }
 
-template <class Phys, int ND, int P, class R, int W>
+template <class Phys, int ND, int P, class R, int W, bool LIM>
 hipError_t launch_stage(const void *args, hipStream_t s) {
   using S = Shape<ND, P, Phys::NC>;
   const StageArgs<R> &a = *static_cast<const StageArgs<R> *>(args);
   const int nb = a.ntiles;
   if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R, W>), dim3(nb), dim3(S::BLOCK), 0, s, a);
+  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R, W, LIM>), dim3(nb), dim3(S::BLOCK), 0, s, a);
   return hipGetLastError();
 }
 

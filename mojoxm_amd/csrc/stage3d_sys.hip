@@ -2,7 +2,7 @@
 #include "stage_families.hpp"
 
 namespace mxm {
-StageLaunch stage_3d_sys_fp64(int phys, int variant, int P, int cfg) {
-  return pick_order<Family3DSystem, double>(phys, variant, P, cfg);
+StageLaunch stage_3d_sys_fp64(int phys, int variant, int P, int cfg, bool lim) {
+  return pick_order<Family3DSystem, double>(phys, variant, P, cfg, lim);
 }
 }  // namespace mxm
