@@ -104,8 +104,9 @@ def test_fp32_error_budget_full_vortex_run():
     """The error bar of the FP32-MFMA form on its benchmark run: the
     128^2 P = 3 vortex to T = 2 (4,054 steps) from the same
     FP32-representable state stays within 1e-5 relative L2 of the plain FP64
-    kernels (observed 2.8e-6 with the stochastically rounded state store,
-    9.67e-6 with round-to-nearest)."""
+    kernels (observed 5.2e-6 with the stochastically rounded state store
+    and the upwind-side HLLEC flux, 3.0e-6 with the wave-sum flux, 9.67e-6
+    with round-to-nearest storage)."""
     pb = M.problems.euler_vortex_2d()
     mesh = M.LocalMesh2D(*pb.n, *pb.L, P=pb.P)
     q0 = pb.ic(mesh.elem_node_xyz).astype(np.float32).astype(np.float64)
