@@ -197,7 +197,10 @@ int mxm_dg_rk_stage(void *solver, int stage, double dt);
 /* nsteps x step_ssprk3, src/solver.mojo:1071-1092.  The steps are launched
  * as replayed HIP graphs, one of 32 steps and one of exactly the remainder
  * nsteps mod 32, rebuilt when dt, the limiter or the launch configuration
- * changed since the last call. */
+ * changed since the last call.  With the limiter on, each stage's factors
+ * are applied by the next stage as it reads its inputs, and the limited
+ * state is written at the end of the call: the state a caller sees is the
+ * one per-stage limiting gives, bit for bit. */
 int mxm_dg_step_ssprk3(void *solver, double dt, int nsteps);
 /* Builds the graphs mxm_dg_step_ssprk3(solver, dt, nsteps) replays, without
  * running a step (no reference counterpart: lets a caller keep graph
