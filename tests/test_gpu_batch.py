@@ -180,12 +180,16 @@ def test_batch_failed_instance_is_switched_off():
             assert np.array_equal(b.get_state(i), alone((6, 5), (1.0, 1.0), phys, P, q, dt, ns, M.PREC_FP64, bcs=bcs))
 
 
+@pytest.mark.parametrize("limiter", [False, True])
 @pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_FP32_MFMA])
-def test_batch_reuses_free_ranges_and_compacts(precision):
+def test_batch_reuses_free_ranges_and_compacts(precision, limiter):
     """A removed instance's tiles are taken by a later instance that fits
     them, with no new device memory (no compaction); one that fits nowhere
     compacts the group to exactly its instances' footprints.  Every
-    instance equals its solve alone, whichever way it was placed."""
+    instance equals its solve alone, whichever way it was placed.  With the
+    limiter, the factors still pending on the instances' states move with
+    them through the compaction, and the new instance in the freed range
+    starts with none of the removed one's."""
     rng = np.random.default_rng(9)
     phys = ph.Euler(flux_type=ph.FLUX_HLLEC)
     dt, P = 1e-3, 3
@@ -197,17 +201,17 @@ def test_batch_reuses_free_ranges_and_compacts(precision):
     sizes = {"a": (12, 10), "b": (6, 6), "c": (8, 5), "d": (6, 5), "e": (16, 16)}
     q = {k: state(v) for k, v in sizes.items()}
     steps = {"a": 9, "b": 4, "c": 7, "d": 5, "e": 3}
-    fp = {k: footprint(v, (1.0, 1.0), phys, P, precision) for k, v in sizes.items()}
+    fp = {k: footprint(v, (1.0, 1.0), phys, P, precision, limiter) for k, v in sizes.items()}
     with Batch(precision) as b:
-        ids = {k: b.add(sizes[k], (1.0, 1.0), phys, P, q[k], dt, steps[k]) for k in "abc"}
+        ids = {k: b.add(sizes[k], (1.0, 1.0), phys, P, q[k], dt, steps[k], limiter=limiter) for k in "abc"}
         b.step(2)
         m1 = b.memory()
         assert fp["a"] + fp["b"] + fp["c"] <= m1 <= fp["a"] + fp["b"] + fp["c"] + 64 * 1024
         b.remove(ids.pop("b"))
-        ids["d"] = b.add(sizes["d"], (1.0, 1.0), phys, P, q["d"], dt, steps["d"])  # fits b's tiles
+        ids["d"] = b.add(sizes["d"], (1.0, 1.0), phys, P, q["d"], dt, steps["d"], limiter=limiter)  # b's tiles
         b.step(3)
         assert b.memory() == m1
-        ids["e"] = b.add(sizes["e"], (1.0, 1.0), phys, P, q["e"], dt, steps["e"])  # fits nowhere: compaction
+        ids["e"] = b.add(sizes["e"], (1.0, 1.0), phys, P, q["e"], dt, steps["e"], limiter=limiter)  # compaction
         b.step(2)
         m2 = b.memory()
         assert fp["a"] + fp["c"] + fp["d"] + fp["e"] <= m2 <= fp["a"] + fp["c"] + fp["d"] + fp["e"] + 64 * 1024
@@ -215,7 +219,10 @@ def test_batch_reuses_free_ranges_and_compacts(precision):
         assert set(b.poll()) == set(ids.values())
         for k, i in ids.items():
             assert np.array_equal(b.get_state(i), alone(sizes[k], (1.0, 1.0), phys, P, q[k], dt, steps[k],
-                                                        precision)), k
+                                                        precision, limiter=limiter)), k
+    if limiter:  # the limiter acted on these states
+        args = (sizes["a"], (1.0, 1.0), phys, P, q["a"], dt, steps["a"], precision)
+        assert not np.array_equal(alone(*args, limiter=True), alone(*args))
 
 
 def test_batch_lagged_poll_reports_everything():
