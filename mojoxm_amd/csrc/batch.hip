@@ -95,10 +95,12 @@ struct Arena {
   DevBuf<double> qd[3];
   DevBuf<float> qf[3];
   // limiter (groups with it): cell means [3][slots*NC] and factors
-  // [3][slots] of the stage buffers, node extrema [slots*NC][2]
+  // [3][slots] of the stage buffers, node extrema [slots*NCL][2] (the
+  // limited components: all NC in 3D, component 0 in 2D)
   DevBuf<double> lmean, ltheta, lext;
   DevBuf<double> invJ, inv_vol, jgeo, iprm, idt, ipost;
-  DevBuf<int4> conn, jobs;
+  DevBuf<int> conn;  // limiter: partner slot per face
+  DevBuf<int4> jobs;
   DevBuf<unsigned short> ipr;
   DevBuf<int> tile_count, tile_inst, iactive, isteps, insteps, ifail;
   void swap(Arena &o) noexcept {
@@ -238,11 +240,11 @@ size_t instance_bytes(const ElemTables &t, int NP, int NC, int NFP, int dim, int
   const size_t word = prec == MXM_PREC_FP64 ? sizeof(double) : sizeof(float);
   size_t b = 3 * slots * NP * NC * word;                            // state trio
   b += slots * (dim * dim + 1) * sizeof(double);                    // invJ, inv_vol
-  b += slots * (dim + 1) * sizeof(int4);                            // limiter neighbours
+  b += slots * (dim + 1) * sizeof(int);                             // limiter neighbours
   b += js * (sizeof(int4) + 4 * sizeof(double) + NFP * sizeof(unsigned short));  // jobs, geometry, partner nodes
   b += tiles * 2 * sizeof(int);                                     // tile_count, tile_inst
   b += (MXM_PRM_MAX + 2) * sizeof(double) + 4 * sizeof(int);        // per-instance scalars
-  if (limiter) b += slots * (5 * NC + 3) * sizeof(double);          // means, factors, extrema
+  if (limiter) b += slots * (3 * NC + 2 * (dim == 3 ? NC : 1) + 3) * sizeof(double);  // means, factors, extrema
   return b;
 }
 
@@ -250,7 +252,8 @@ size_t instance_bytes(const ElemTables &t, int NP, int NC, int NFP, int dim, int
 // tile bt, slot sl, maxj job slots per tile), in the arena's layouts.
 struct InstImage {
   std::vector<double> ij, iv, geo, prm;
-  std::vector<int4> cn, jb;
+  std::vector<int> cn;
+  std::vector<int4> jb;
   std::vector<unsigned short> pr;
   std::vector<int> tc, ti;
   double dt = 0.0, post = 1.0;
@@ -266,11 +269,7 @@ InstImage inst_image(const BGroup &g, const BInst &in, int bt, int sl, int maxj)
   m.ij.assign(t.invJ.begin(), t.invJ.begin() + slots * dd);
   m.iv.assign(t.inv_vol.begin(), t.inv_vol.begin() + slots);
   m.cn.resize(slots * NF);
-  for (size_t s = 0; s < slots * NF; ++s) {
-    int4 c = t.conn[s];
-    c.x += bs;
-    m.cn[s] = c;
-  }
+  for (size_t s = 0; s < slots * NF; ++s) m.cn[s] = t.conn[s] + bs;
   m.jb.resize((size_t)lt * maxj);
   m.geo.assign((size_t)lt * maxj * 4, 0.0);
   m.pr.assign((size_t)lt * maxj * NFP, 0);
@@ -511,7 +510,7 @@ struct Batch {
         {q[0].p + bs * g.NP * g.NC, qs.data(), qs.size() * sizeof(R)},
         {g.ar.invJ.p + bs * dd, m.ij.data(), m.ij.size() * sizeof(double)},
         {g.ar.inv_vol.p + bs, m.iv.data(), m.iv.size() * sizeof(double)},
-        {g.ar.conn.p + bs * NF, m.cn.data(), m.cn.size() * sizeof(int4)},
+        {g.ar.conn.p + bs * NF, m.cn.data(), m.cn.size() * sizeof(int)},
         {g.ar.jobs.p + (size_t)bt * g.maxj, m.jb.data(), m.jb.size() * sizeof(int4)},
         {g.ar.jgeo.p + (size_t)bt * g.maxj * 4, m.geo.data(), m.geo.size() * sizeof(double)},
         {g.ar.ipr.p + (size_t)bt * g.maxj * g.NFP, m.pr.data(), m.pr.size() * sizeof(unsigned short)},
@@ -648,13 +647,14 @@ struct Batch {
     else
       state_trio(na.qf, g.ar.qf);
     std::vector<double> ij(slots * dd, 0.0), iv(slots, 0.0), geo((size_t)cap_tiles * maxj * 4, 0.0);
-    std::vector<int4> cn(slots * NF), jb((size_t)cap_tiles * maxj);
+    std::vector<int> cn(slots * NF);
+    std::vector<int4> jb((size_t)cap_tiles * maxj);
     std::vector<unsigned short> pr((size_t)cap_tiles * maxj * NFP, 0);
     std::vector<int> tc(cap_tiles, 0), ti(cap_tiles, 0);
     std::vector<double> iprm((size_t)cap_slots * MXM_PRM_MAX, 0.0), idt(cap_slots, 0.0), ipost(cap_slots, 1.0);
     std::vector<int> iact(cap_slots, 0), ist(cap_slots, 0), inst_n(cap_slots, 0), ifl(cap_slots, 0);
     for (int s = 0; s < (int)slots; ++s)
-      for (int lf = 0; lf < NF; ++lf) cn[(size_t)s * NF + lf] = int4{s, 0, 0, 0};
+      for (int lf = 0; lf < NF; ++lf) cn[(size_t)s * NF + lf] = s;
     for (int T = 0; T < cap_tiles; ++T)
       for (int j = 0; j < maxj; ++j) jb[(size_t)T * maxj + j] = int4{-1, T * epb, 0, 0};
     std::vector<int> new_base(ni);
@@ -694,7 +694,7 @@ struct Batch {
     na.ifail.upload(ifl.data(), ifl.size());
     if (g.limiter) {
       na.lmean.alloc(3 * slots * g.NC);
-      na.lext.alloc(2 * slots * g.NC);
+      na.lext.alloc(2 * slots * (g.dim == 3 ? g.NC : 1));
       na.ltheta.alloc(3 * slots);
       MXM_HIP(launch_fill(na.ltheta.p, (long long)na.ltheta.n, 1.0, g.stream));
     }

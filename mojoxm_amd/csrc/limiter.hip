@@ -38,7 +38,7 @@ namespace {
 // instances keep their theta
 template <int NC, bool ALL>
 __global__ __launch_bounds__(256) void bj_theta_kernel(double *__restrict__ theta, const double *__restrict__ mean,
-                                                       const double *__restrict__ ext, const int4 *__restrict__ conn,
+                                                       const double *__restrict__ ext, const int *__restrict__ conn,
                                                        int K, int NF, double eps2, const int *__restrict__ tile_count,
                                                        const int *__restrict__ tile_inst,
                                                        const int *__restrict__ inst_active, int tepb) {
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void bj_theta_kernel(double *__restrict__ thet
     const double bm = mean[(size_t)e * NC + c];
     double mn = bm, mx = bm;
     for (int lf = 0; lf < NF; ++lf) {
-      const int p = conn[(size_t)e * NF + lf].x;
+      const int p = conn[(size_t)e * NF + lf];
       if (!ALL && p == e) continue;  // 2D: boundary faces do not constrain
       const double v = mean[(size_t)p * NC + c];
       mn = v < mn ? v : mn;
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void bj_theta_kernel(double *__restrict__ thet
     const double nD = bm - mn < 0.0 ? 0.0 : bm - mn;
 #pragma unroll
     for (int s = 0; s < 2; ++s) {  // the smallest and the largest node value
-      const double d = ext[((size_t)e * NC + c) * 2 + s] - bm;
+      const double d = ext[((size_t)e * NCL + c) * 2 + s] - bm;
       const double da = d >= 0.0 ? d : -d;
       if (da <= 1.0e-30) continue;
       const double D = d > 0.0 ? pD : nD;
@@ -104,7 +104,7 @@ __global__ void fill_kernel(double *p, long long n, double v) {
 }
 
 template <int NC>
-void theta_nc(double *theta, const double *mean, const double *ext, const int4 *conn, int K, int NF, bool all,
+void theta_nc(double *theta, const double *mean, const double *ext, const int *conn, int K, int NF, bool all,
               double eps2, hipStream_t s, const int *tile_count, const int *tile_inst, const int *inst_active,
               int tepb) {
   const dim3 grid((K + 255) / 256);
@@ -124,7 +124,7 @@ void apply_nc(R *q, double *theta, const double *mean, int K, int NP, hipStream_
 
 }  // namespace
 
-hipError_t launch_bj_theta(double *theta, const double *mean, const double *ext, const int4 *conn, int K, int NC,
+hipError_t launch_bj_theta(double *theta, const double *mean, const double *ext, const int *conn, int K, int NC,
                            int NF, int dim, double eps, hipStream_t s, const int *tile_count, int epb_tiles,
                            const int *tile_inst, const int *inst_active) {
   if (K <= 0) return hipSuccess;

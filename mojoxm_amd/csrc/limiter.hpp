@@ -9,14 +9,15 @@
 namespace mxm {
 constexpr int kLimMaxNC = 17;  // the widest state (two-fluid)
 // theta[K]: one factor per element slot from mean [K*NC] (sum_i w_i q_i),
-// ext [K*NC][2] (smallest, largest node value) and the neighbours' means
-// through conn (element-centric connectivity, partner in .x); FP64
+// ext [K*NCL][2] (smallest, largest node value of the limited components:
+// all NC in 3D, component 0 in 2D) and the neighbours' means through conn
+// (the partner slot of each of an element's NF faces); FP64
 // whatever the state precision.  tile_count[slot / epb]: real elements of
 // each tile of epb slots (the others are padding and keep their theta).  In
 // a batch, tile_inst[slot / epb] names the instance of a slot and elements
 // of instances with inst_active == 0 keep theirs (both null for a single
 // solver).
-hipError_t launch_bj_theta(double *theta, const double *mean, const double *ext, const int4 *conn, int K, int NC,
+hipError_t launch_bj_theta(double *theta, const double *mean, const double *ext, const int *conn, int K, int NC,
                            int NF, int dim, double eps, hipStream_t s, const int *tile_count, int epb,
                            const int *tile_inst = nullptr, const int *inst_active = nullptr);
 // q = mean + theta (q - mean) in place for every slot with theta < 1, then

@@ -198,7 +198,7 @@ struct ElemTables {
   Tiling til;
   FaceJobs fj;
   std::vector<double> invJ, inv_vol;
-  std::vector<int4> conn;  // limiter neighbours {partner slot, face, flags, 0}
+  std::vector<int> conn;  // limiter neighbours: the partner slot of each face (its own slot on a boundary)
 };
 
 inline ElemTables build_elem_tables(const HostMesh &m, const RefElem &re, int nc) {
@@ -215,7 +215,7 @@ inline ElemTables build_elem_tables(const HostMesh &m, const RefElem &re, int nc
     const int e = t.til.ref_of[sl];
     for (int lf = 0; lf < NF; ++lf) {
       const int32_t *c = &cn.conn[((size_t)(e < 0 ? 0 : e) * NF + lf) * 4];
-      t.conn[(size_t)sl * NF + lf] = e < 0 ? int4{sl, 0, 0, 0} : int4{t.til.dev_of[c[0]], c[1], c[2], 0};
+      t.conn[(size_t)sl * NF + lf] = e < 0 ? sl : t.til.dev_of[c[0]];
     }
     if (e < 0) continue;
     for (int k = 0; k < dd; ++k) t.invJ[(size_t)sl * dd + k] = m.elem_invJ[(size_t)e * dd + k];

@@ -53,12 +53,13 @@ struct Solver {
   DevBuf<double> q, q1, q2;
   DevBuf<float> f, f1, f2;
   // limiter: cell means [3][Kd*NC] and factors [3][Kd] of the stage
-  // buffers (q, q1, q2), node extrema [Kd*NC][2] of the last stage output
+  // buffers (q, q1, q2), node extrema [Kd*NCL][2] of the last stage output
+  // (the limited components: all NC in 3D, component 0 in 2D)
   DevBuf<double> lmean, ltheta, lext;
   DevBuf<double> invJ, inv_vol, jgeo, Dt, Lt, w, dprm;
   DevBuf<double> Dn, dpart, dout;  // diagnostics: nodal derivative, tile partials, results
   DevBuf<float> Aop;
-  DevBuf<int4> conn;
+  DevBuf<int> conn;  // limiter: partner slot per face
   DevBuf<int> fmap, tile_count;
   DevBuf<int4> jobs;
   DevBuf<unsigned short> ipr;
@@ -131,7 +132,7 @@ struct Solver {
       }
     }
     lmean.alloc((size_t)3 * Kd * NC);
-    lext.alloc((size_t)2 * Kd * NC);
+    lext.alloc((size_t)2 * Kd * (dim == 3 ? NC : 1));  // the limited components only
     ltheta.alloc((size_t)3 * Kd);
     MXM_HIP(launch_fill(ltheta.p, (long long)ltheta.n, 1.0, stream));
     invJ.upload(t.invJ.data(), t.invJ.size());

@@ -67,7 +67,8 @@ struct StageArgs {
   const int *inst_active;   // [n_inst] or nullptr (all active)
   int *inst_fail;           // [n_inst] or nullptr: set to 1 when a stored value is not finite
   double *mean;             // [slots][NC] or nullptr: cell means sum_i w_i q_out_i of the stored state
-  double *ext;              // [slots][NC][2] (with mean): smallest and largest stored node value
+  double *ext;              // [slots][NCL][2] (with mean): smallest and largest stored node value of the
+                            // limited components (all NC in 3D, component 0 in 2D)
   const double *w;          // [NP] node weights (with mean)
   // Limiter factors pending on the input buffers (nullptr: none): every
   // value read from q_in (tile and partner traces) or q_a is used as
@@ -729,7 +730,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
 #pragma unroll
       for (int n = 0; n < NP; ++n) sm += A.w[n] * double(sW[(el * NP + n) * NC + c]);
       A.mean[(size_t)(e0 + el) * NC + c] = sm;
-      {
+      constexpr int NCL = ND == 3 ? NC : 1;  // the components the limiter bounds
+      if (c < NCL) {
         double lo = double(sW[el * NP * NC + c]), hi = lo;
 #pragma unroll
         for (int n = 1; n < NP; ++n) {
@@ -737,8 +739,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
           lo = v < lo ? v : lo;
           hi = v > hi ? v : hi;
         }
-        A.ext[((size_t)(e0 + el) * NC + c) * 2] = lo;
-        A.ext[((size_t)(e0 + el) * NC + c) * 2 + 1] = hi;
+        A.ext[((size_t)(e0 + el) * NCL + c) * 2] = lo;
+        A.ext[((size_t)(e0 + el) * NCL + c) * 2 + 1] = hi;
       }
     }
   }
