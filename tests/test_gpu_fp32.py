@@ -141,3 +141,31 @@ def test_fp32_error_budget_largest_alfven_instances():
                 s.step_ssprk3(m.dt, m.nsteps)
                 out[prec] = s.get_state()
         assert rel_l2(out[M.PREC_FP32_MFMA], out[M.PREC_FP64]) < TOL32, m.index
+
+
+def test_fp32_stored_state_rounding_is_unbiased():
+    """The stochastic rounding of the FP32 form's stored state keeps the
+    expected value.  A uniform 2D Euler state (rho = 1, u = (1, 0), p = 1)
+    under a uniform gravity g along x: the fluxes stay uniform, so
+    m_x(t) = 1 + g t exactly.  With dt g a quarter of a float ulp of m_x,
+    round-to-nearest storage (or truncation) would keep m_x at exactly 1,
+    and a draw biased upward would add an ulp per store; the mean of m_x
+    over the nodes follows the FP64 run (measured 1.6 % above it, for a
+    500-ulp rise).  The 10 % bound covers the FP32 operator's own drift on
+    the uniform state (its volume and surface sums cancel only to FP32
+    rounding: -7.8e-7 in m_x over the same run with g = 0, 1.3 % of the
+    rise) and the random walk of ten node classes (the salt depends on the
+    node only)."""
+    g, dt, nsteps = 3.0e-5, 1.0e-3, 2000
+    phys = ph.Euler(gx=g)
+    n = (8, 8)
+    rise = {}
+    for prec in (M.PREC_FP64, M.PREC_FP32_MFMA):
+        with M.Solver(n, (1.0, 1.0), phys, P=3, precision=prec) as s:
+            q = np.zeros(s.shape)
+            q[..., 0], q[..., 1], q[..., 4] = 1.0, 1.0, 1.0 / 0.4 + 0.5
+            s.set_state(q)
+            s.step_ssprk3(dt, nsteps)
+            rise[prec] = s.get_state()[..., 1].mean() - 1.0
+    assert rise[M.PREC_FP64] == pytest.approx(g * dt * nsteps, rel=1e-6)
+    assert rise[M.PREC_FP32_MFMA] == pytest.approx(rise[M.PREC_FP64], rel=0.1)
