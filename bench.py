@@ -52,6 +52,12 @@ LARGE_PROFILE = os.path.join(ROOT, "profiles", "dg_stage_kernel_euler_vortex_2d_
 CFG_WAVES = {"fp64": ["compiler", "5", "4"], "fp32_mfma": ["compiler", "5", "6", "7"]}
 
 
+def tile_report(s):
+    """The tiling the autotuner kept: elements per tile, threads per block."""
+    wide, elems, threads = s.tile
+    return {"elements": elems, "threads": threads, "wide": wide}
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
@@ -173,6 +179,7 @@ def advection_leg(M, args):
             U = s.dof_count()
             s.set_state(q0)
             cfg, _ = s.autotune(pb.dt, 20)
+            tile = tile_report(s)
             s.set_state(q0)
             s.prepare_steps(pb.dt, pb.nsteps)
             s.synchronize()
@@ -186,7 +193,8 @@ def advection_leg(M, args):
         alg = 8.0 / 3.0 * U * word
         res[name] = {"value": 3.0 * U * pb.nsteps / wall, "wall_s": wall, "kernel_avg_us": avg_us,
                      "achieved_gbs": alg / (avg_us * 1e-6) / 1e9, "frac": alg / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                     "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg]},
+                     "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg],
+                                       "tile": tile},
                      "l2_vs_initial_T1": rel_l2(finals[name], q0)}
     out = {"workload": "advection_gaussian_3d: 16^3 x 6 tets, P=2, NC=1, upwind, CFL 0.3, 462 steps to T=1",
            "dof_per_stage": U, "dt": pb.dt, "steps": pb.nsteps, "fp64": res["fp64"], "fp32_mfma": res["fp32_mfma"]}
@@ -253,6 +261,7 @@ def large_vortex_leg(M, args, nsteps=None):
             U = s.dof_count()
             s.set_state(q0)
             cfg, cfg_us = s.autotune(pb.dt, 10)
+            tile = tile_report(s)
             s.set_state(q0)
             if nsteps:
                 s.step_ssprk3(pb.dt, nsteps)
@@ -263,7 +272,8 @@ def large_vortex_leg(M, args, nsteps=None):
         out[name] = {"kernel_avg_us": avg_us, "algorithmic_bytes_per_launch": alg, "achieved_gbs": gbs,
                      "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic(name, LARGE_PROFILE),
                      "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg],
-                                       "us_per_config": dict(zip(CFG_WAVES[name], cfg_us))}}
+                                       "us_per_config": dict(zip(CFG_WAVES[name], cfg_us)),
+                                       "tile": tile}}
     out["dof_per_stage"] = U
     out["speedup_fp32_mfma_vs_fp64"] = out["fp64"]["kernel_avg_us"] / out["fp32_mfma"]["kernel_avg_us"]
     out["traffic_from"] = os.path.relpath(LARGE_PROFILE, ROOT)
@@ -291,6 +301,7 @@ def profile_limited(M, args) -> int:
             s.enable_cell_limiter(True, 0.1)
             s.set_state(q0)
             cfg, _ = s.autotune(dt, 10)
+            tile = tile_report(s)
             s.set_state(q0)
             s.prepare_steps(dt, args.profile_limited)
             s.synchronize()
@@ -298,7 +309,7 @@ def profile_limited(M, args) -> int:
             s.step_ssprk3(dt, args.profile_limited)
             s.synchronize()
             out[name] = {"ms_per_step": (time.perf_counter() - t0) / args.profile_limited * 1e3,
-                         "launch_config": CFG_WAVES[name][cfg], "dof_per_stage": s.dof_count(),
+                         "launch_config": CFG_WAVES[name][cfg], "tile": tile, "dof_per_stage": s.dof_count(),
                          "finite": bool(np.isfinite(s.get_state()).all())}
     print(json.dumps(out))
     return 0
@@ -319,6 +330,7 @@ def taylor_green_leg(M, args):
             U = s.dof_count()
             s.set_state(q0)
             cfg, cfg_us = s.autotune(pb.dt, 10)
+            tile = tile_report(s)
             s.set_state(q0)
             s.prepare_steps(pb.dt, pb.nsteps)
             s.synchronize()
@@ -341,7 +353,8 @@ def taylor_green_leg(M, args):
         res[name] = {"value": 3.0 * U * pb.nsteps / wall, "ms_per_step": wall / pb.nsteps * 1e3,
                      "kernel_avg_us": avg_us, "achieved_gbs": alg / (avg_us * 1e-6) / 1e9,
                      "frac": alg / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-                     "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg]},
+                     "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg],
+                                       "tile": tile},
                      "ke_enstrophy_every_20_steps": [[k, ke, en] for k, ke, en in hist]}
     h64 = res["fp64"]["ke_enstrophy_every_20_steps"]
     h32 = res["fp32_mfma"]["ke_enstrophy_every_20_steps"]
@@ -503,6 +516,7 @@ def main() -> int:
         # launch configuration: every occupancy target timed on this mesh
         solver.set_state(q0)
         cfg, cfg_us = solver.autotune(pb.dt, 30)
+        tile = tile_report(solver)
         # ---- timed region: K steps with the state resident in HBM --------
         solver.set_state(q0)
         solver.step_ssprk3(pb.dt, args.warmup)
@@ -522,7 +536,8 @@ def main() -> int:
                           "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(name),
                           "kernel_avg_us": avg_us, "algorithmic_bytes_per_launch": bytes_per_launch},
              "launch_config": {"chosen": cfg, "waves_per_simd": CFG_WAVES[name][cfg],
-                               "us_per_config": dict(zip(CFG_WAVES[name], cfg_us))}}
+                               "us_per_config": dict(zip(CFG_WAVES[name], cfg_us)),
+                               "tile": tile}}
         # ---- accuracy of the full run to T=2 ------------------------------
         if not args.no_accuracy:
             solver.set_state(q0)

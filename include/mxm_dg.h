@@ -260,16 +260,26 @@ int mxm_dg_max_wave_speed_noalloc(void *solver, double *speed_dev);
  * launch shape per kernel).  Each configuration of the stage kernel is an
  * occupancy target (waves per SIMD) that bounds its register allocation;
  * all compute the same bits.  mxm_dg_autotune times each on this solver's
- * mesh -- one SSPRK3 stage 1 into the first work buffer, then reps launches
- * of stage 2, which reads it and the state and writes the second work
- * buffer, so the state is left unchanged -- and keeps the fastest: *cfg is
- * its index,
- * us[0..cap) the microseconds per launch of each configuration
+ * mesh -- reps untimed SSPRK3 steps of the current state, then per
+ * configuration one untimed step and reps steps between two events; the
+ * state is saved to the host first and restored after, so it is left
+ * unchanged -- and keeps the fastest: *cfg is its
+ * index, us[0..cap) the microseconds per stage of each configuration
  * (mxm_get_constant(2 / 3) = number of configurations of the FP64 /
  * FP32-MFMA form).  mxm_dg_launch_config sets configuration cfg (cfg < 0:
- * no change) and reports the current one. */
+ * no change) and reports the current one.
+ * For a shape with a wide tile (the FP32-MFMA form on triangles at P = 3:
+ * 32 elements on 320 threads instead of 24 on 256), mxm_dg_autotune also
+ * times every configuration on the other tile size and keeps the faster
+ * tiling; the state is moved between the two layouts and keeps its values
+ * (*cfg and us[] are those of the kept tiling). */
 int mxm_dg_autotune(void *solver, double dt, int reps, int *cfg, double *us, int cap);
 int mxm_dg_launch_config(void *solver, int cfg, int *current);
+/* Tile size of the stage kernel (no reference counterpart): wide = 0 lays
+ * the state out in tiles of 256 threads, 1 in wide tiles (MXM_ERR_ARG for
+ * a shape or form without them), -1 changes nothing; the state keeps its
+ * values.  Reports the current tile's elements and threads per block. */
+int mxm_dg_tile_config(void *solver, int wide, int *elems, int *threads);
 int mxm_dg_destroy(void *solver);
 
 /* ---- batch engine ----------------------------------------------------------

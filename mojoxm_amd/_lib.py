@@ -76,6 +76,7 @@ EXPORTS = {
     "mxm_dg_autotune": ([C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.c_int],
                         C.c_int),
     "mxm_dg_launch_config": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
+    "mxm_dg_tile_config": ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_diagnostics": ([C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)], C.c_int),
     "mxm_dg_max_wave_speed": ([C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "mxm_dg_max_wave_speed_noalloc": ([C.c_void_p, C.c_void_p], C.c_int),

@@ -355,6 +355,8 @@ struct Batch {
     const RefElem &re = ref(in.dim, in.P);
     g->dim = in.dim, g->P = in.P, g->phys = in.phys, g->variant = in.variant, g->NC = in.NC, g->NP = re.NP,
     g->NFP = re.NFP, g->NF = re.NF;
+    // 256-thread tiles: an arena's tiles queue for the CUs, where the wide
+    // tile is slower (stage_epb)
     g->epb = stage_epb(in.dim, re.NP, re.NFP, in.NC);
     g->prec = prec;
     g->limiter = in.limiter, g->eps = in.eps;
@@ -415,7 +417,7 @@ struct Batch {
     in->K = m.K, in->NP = m.NP;
     if (!q0 || n != (long long)m.K * m.NP * nc) throw ArgError("initial state length must be K*NP*NC");
     in->q0.assign(q0, q0 + n);
-    in->t = build_elem_tables(m, re, nc);
+    in->t = build_elem_tables(m, re, nc, false);
     in->id = next_id++;
     BGroup &g = group_of_key(*in);
     g.waiting.push_back(in->id);
@@ -1088,7 +1090,7 @@ int footprint(int dim, const void *desc, int P, int phys, int prec, int limiter,
     check_bc(d->bc, 4);
     m = build_mesh_2d(d->nx, d->ny, d->lx, d->ly, d->bc, P, re);
   }
-  ElemTables t = build_elem_tables(m, re, nc);
+  ElemTables t = build_elem_tables(m, re, nc, false);
   *out = (long long)instance_bytes(t, m.NP, nc, m.NFP, dim, prec, limiter != 0);
   return MXM_OK;
 }

@@ -4,9 +4,10 @@
 // (examples/euler_vortex_2d_gpu.mojo:148-154) and the wave speeds its flux
 // hooks return are discarded (src/solver.mojo:229,266,268).
 //
-// One block per tile, one thread per element node (as the stage kernel);
-// each wave reduces with lane shuffles, the block over its 4 waves in LDS,
-// and one block then reduces the tile maxima.  The speed is evaluated in
+// One 256-thread block per tile, each thread over the tile's element
+// nodes tid, tid + 256, ...; each wave reduces with lane shuffles, the
+// block over its 4 waves in LDS, and one block then reduces the tile
+// maxima.  The speed is evaluated in
 // FP64 from the state (FP64 or FP32).  Max is exact, so the result does
 // not depend on the order; a NaN anywhere gives a NaN result.
 #include <hip/hip_runtime.h>
@@ -33,10 +34,11 @@ __global__ __launch_bounds__(256) void speed_tile_kernel(SpeedArgs<S> A) {
   const int T = A.tile0 + blockIdx.x;
   const int nE = A.tile_count[T];
   double sp = 0.0;
-  if (tid < nE * A.NP) {
-    const int inst = A.tile_inst ? A.tile_inst[T] : 0;
-    const Phys phys = Phys::from(A.prm + (size_t)inst * kPrmMax);
-    const S *g = A.q + ((size_t)T * A.epb * A.NP + tid) * NC;
+  const int inst = A.tile_inst ? A.tile_inst[T] : 0;
+  const Phys phys = Phys::from(A.prm + (size_t)inst * kPrmMax);
+  // a tile can hold more nodes than the block has threads (up to 512)
+  for (int t = tid; t < nE * A.NP; t += 256) {
+    const S *g = A.q + ((size_t)T * A.epb * A.NP + t) * NC;
     double q[NC];
     bool bad = false;
 #pragma unroll
@@ -45,7 +47,7 @@ __global__ __launch_bounds__(256) void speed_tile_kernel(SpeedArgs<S> A) {
       bad = bad || q[c] != q[c];
     }
     // the physics' floors would replace a NaN density; report it instead
-    sp = bad ? __builtin_nan("") : phys.max_speed(q);
+    sp = nan_max(sp, bad ? __builtin_nan("") : phys.max_speed(q));
   }
   sp = wave_nan_max(sp);
   if (lane == 0) sW[wave] = sp;

@@ -15,9 +15,10 @@
 // by invJ (the quantities the Taylor-Green driver approximates from
 // momentum squares, examples/euler_taylor_green.mojo:181-195).
 //
-// Two kernels: one block per tile (one thread per element node, as the
-// stage kernel) reduces its tile to a compensated pair (s, e) per sum and
-// a max per component; one block then reduces the tile partials.
+// Two kernels: one block per tile (one thread per element node: 256
+// threads, 512 for tiles of more nodes) reduces its tile to a compensated
+// pair (s, e) per sum and a max per component; one block then reduces the
+// tile partials.
 #include <hip/hip_runtime.h>
 
 #include "diag.hpp"
@@ -57,13 +58,14 @@ __device__ __forceinline__ double wave_max(double m) {
   return m;
 }
 
-template <int ND, int P, class R>
-__global__ __launch_bounds__(256) void diag_tile_kernel(DiagArgs<R> A) {
-  constexpr int NP = Shape<ND, P>::NP;
+// B threads (256, or 512 for tiles of more than 256 nodes)
+template <int ND, int P, class R, int B>
+__global__ __launch_bounds__(B) void diag_tile_kernel(DiagArgs<R> A) {
+  constexpr int NP = Shape<ND, P>::NP, NW = B / 64;
   constexpr double kRefVol = ND == 3 ? 1.0 / 6.0 : 0.5;
-  __shared__ double sU[256 * 3];
-  __shared__ double sPart[4][kDiagSums][2];
-  __shared__ double sMax[4][kDiagMaxNC];
+  __shared__ double sU[B * 3];
+  __shared__ double sPart[NW][kDiagSums][2];
+  __shared__ double sMax[NW][kDiagMaxNC];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int T = A.tile0 + blockIdx.x;
   const int nE = A.tile_count[T];
@@ -130,12 +132,14 @@ __global__ __launch_bounds__(256) void diag_tile_kernel(DiagArgs<R> A) {
   __syncthreads();
   if (tid < ns) {
     double s = sPart[0][tid][0], e = sPart[0][tid][1];
-    for (int w = 1; w < 4; ++w) add_pair(s, e, sPart[w][tid][0], sPart[w][tid][1]);
+    for (int w = 1; w < NW; ++w) add_pair(s, e, sPart[w][tid][0], sPart[w][tid][1]);
     A.part[((size_t)blockIdx.x * kDiagSums + tid) * 2] = s;
     A.part[((size_t)blockIdx.x * kDiagSums + tid) * 2 + 1] = e;
   } else if (tid >= 64 && tid < 64 + NC) {
     const int c = tid - 64;
-    A.pmax[(size_t)blockIdx.x * kDiagMaxNC + c] = fmax(fmax(sMax[0][c], sMax[1][c]), fmax(sMax[2][c], sMax[3][c]));
+    double m = sMax[0][c];
+    for (int w = 1; w < NW; ++w) m = fmax(m, sMax[w][c]);
+    A.pmax[(size_t)blockIdx.x * kDiagMaxNC + c] = m;
   }
 }
 
@@ -173,7 +177,12 @@ __global__ __launch_bounds__(256) void diag_final_kernel(DiagArgs<R> A) {
 
 template <int ND, int P, class R>
 hipError_t launch_dp(const DiagArgs<R> &a, hipStream_t s) {
-  hipLaunchKernelGGL((diag_tile_kernel<ND, P, R>), dim3(a.ntiles), dim3(256), 0, s, a);
+  const int nodes = a.epb * Shape<ND, P>::NP;
+  if (nodes > 512) return hipErrorInvalidValue;
+  if (nodes > 256)
+    hipLaunchKernelGGL((diag_tile_kernel<ND, P, R, 512>), dim3(a.ntiles), dim3(512), 0, s, a);
+  else
+    hipLaunchKernelGGL((diag_tile_kernel<ND, P, R, 256>), dim3(a.ntiles), dim3(256), 0, s, a);
   hipLaunchKernelGGL((diag_final_kernel<R>), dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }

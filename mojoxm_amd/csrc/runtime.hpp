@@ -118,17 +118,20 @@ inline int stage_cfg_count(int prec) { return prec == MXM_PREC_FP64 ? StageCfg<d
 inline int stage_cfg_default(int prec) { return prec == MXM_PREC_FP64 ? StageCfg<double>::dflt : StageCfg<float>::dflt; }
 
 // lim: the kernel variant for a limited run (limiter epilogue and the
-// inputs' pending factors applied on load)
-inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec, int cfg, bool lim = false) {
+// inputs' pending factors applied on load); wide: the kernel of the wide
+// tile (stage_epb), which reads a state laid out in wide tiles
+inline StageLaunch find_stage(int dim, int phys, int variant, int P, int prec, int cfg, bool lim = false,
+                              bool wide = false) {
   const bool f64 = prec == MXM_PREC_FP64;
   if (cfg < 0 || cfg >= stage_cfg_count(prec)) return nullptr;
   if (dim == 3) {
-    if (phys <= MXM_PHYS_EULER) return (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg, lim);
-    return phys >= MXM_PHYS_SW ? (f64 ? stage_3d_sys_fp64 : stage_3d_sys_fp32)(phys, variant, P, cfg, lim) : nullptr;
+    if (phys <= MXM_PHYS_EULER) return (f64 ? stage_3d_fp64 : stage_3d_fp32)(phys, variant, P, cfg, lim, wide);
+    return phys >= MXM_PHYS_SW ? (f64 ? stage_3d_sys_fp64 : stage_3d_sys_fp32)(phys, variant, P, cfg, lim, wide)
+                               : nullptr;
   }
   if (phys >= MXM_PHYS_SW) return nullptr;
-  if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P, cfg, lim);
-  return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P, cfg, lim);
+  if (phys <= MXM_PHYS_EULER) return (f64 ? stage_2d_trait_fp64 : stage_2d_trait_fp32)(phys, variant, P, cfg, lim, wide);
+  return (f64 ? stage_2d_module_fp64 : stage_2d_module_fp32)(phys, variant, P, cfg, lim, wide);
 }
 
 // Times `launch` over `reps` back-to-back launches (after one untimed
@@ -147,10 +150,11 @@ inline double time_launches(StageLaunch launch, const void *args, hipStream_t st
   return (double)ms * 1e3 / reps;
 }
 
-// The autotuner: times every launch configuration of one stage launch
-// (its arguments must leave the launch's inputs unchanged, e.g. stage 1
-// of SSPRK3, which writes q1 from q) and returns the fastest; us[k] gets
-// configuration k's time when us is not null.
+// The batch arena's autotuner: times every launch configuration of one
+// stage launch (its arguments must leave the launch's inputs unchanged,
+// e.g. stage 1 of SSPRK3, which writes q1 from q) and returns the
+// fastest; us[k] gets configuration k's time when us is not null.  (A
+// solver times whole steps of its state instead: Solver::tune.)
 inline int autotune_stage(int dim, int phys, int variant, int P, int prec, const void *args, hipStream_t stream,
                           int reps, double *us, bool lim = false) {
   int best = stage_cfg_default(prec);
@@ -201,10 +205,11 @@ struct ElemTables {
   std::vector<int> conn;  // limiter neighbours: the partner slot of each face (its own slot on a boundary)
 };
 
-inline ElemTables build_elem_tables(const HostMesh &m, const RefElem &re, int nc) {
+// wide: tiles of the wide tile size (stage_epb)
+inline ElemTables build_elem_tables(const HostMesh &m, const RefElem &re, int nc, bool wide) {
   ElemTables t;
   Connectivity cn = build_connectivity(m, re);
-  t.til = build_tiling(m, stage_epb(m.dim, m.NP, m.NFP, nc), m.NFP);
+  t.til = build_tiling(m, stage_epb(m.dim, m.NP, m.NFP, nc, wide), m.NFP);
   const int Kd = t.til.slots(), dd = m.dim * m.dim, NF = m.NF;
   t.invJ.assign((size_t)Kd * dd, 0.0);
   t.inv_vol.assign((size_t)Kd, 0.0);

@@ -45,6 +45,12 @@ struct Solver {
   double prm[MXM_PRM_MAX] = {0};
   int prec = MXM_PREC_FP64;
   int variant = 0, cfg = 0;  // flux variant; launch configuration (StageCfg)
+  // wide tiles (stage_epb): the FP32-MFMA form of a shape that has them,
+  // chosen by autotune() or set by mxm_dg_tile_config; the host mesh is
+  // kept to lay the tables out again
+  bool wide = false;
+  std::unique_ptr<HostMesh> hmesh;
+  std::unique_ptr<RefElem> hre;
   StageLaunch launch = nullptr;
   hipStream_t stream = nullptr;
   bool limiter = false;
@@ -112,13 +118,39 @@ struct Solver {
     MXM_HIP(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
   }
 
-  void build(const HostMesh &m, const RefElem &re) {
+  void build(HostMesh m, const RefElem &re) {
     dim = m.dim, P = m.P, NP = m.NP, NFP = m.NFP, NF = m.NF, K = m.K, Nf = m.Nf;
-    ElemTables t = build_elem_tables(m, re, NC);
+    MXM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    layout(m, re);
+    OpTables o = build_operators(re, dim, NC, prec);
+    if (prec == MXM_PREC_FP64) {
+      Dt.upload(o.Dt.data(), o.Dt.size());
+      Lt.upload(o.Lt.data(), o.Lt.size());
+    } else {
+      Aop.upload(o.Aop.data(), o.Aop.size());
+    }
+    w.upload(re.w.data(), re.w.size());
+    Dn.upload(re.Dn.data(), re.Dn.size());
+    fmap.upload(re.fmap.data(), re.fmap.size());
+    dprm.upload(prm, MXM_PRM_MAX);
+    MXM_HIP(hipDeviceSynchronize());
+    if (has_wide()) {
+      hmesh = std::make_unique<HostMesh>(std::move(m));
+      hre = std::make_unique<RefElem>(re);
+    }
+  }
+
+  bool has_wide() const { return prec == MXM_PREC_FP32_MFMA && stage_has_wide(dim, NP, NFP, NC); }
+  StageLaunch find(int c, bool lim) const { return find_stage(dim, phys, variant, P, prec, c, lim, wide); }
+
+  // the tables and buffers that depend on the tiling (tile size `wide`),
+  // with a zero state and nothing pending; graphs are dropped
+  void layout(const HostMesh &m, const RefElem &re) {
+    drop_graphs();
+    ElemTables t = build_elem_tables(m, re, NC, wide);
     til = std::move(t.til);
     Kd = til.slots();
     ntiles = til.ntiles;
-    MXM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     const size_t nq = (size_t)Kd * NP * NC;
     if (prec == MXM_PREC_FP64) {
       for (DevBuf<double> *b : {&q, &q1, &q2}) {
@@ -143,18 +175,30 @@ struct Solver {
     jobs.upload(reinterpret_cast<const int4 *>(t.fj.jobs.data()), t.fj.jobs.size() / 4);
     jgeo.upload(t.fj.geo.data(), t.fj.geo.size());
     ipr.upload(t.fj.ipr.data(), t.fj.ipr.size());
-    OpTables o = build_operators(re, dim, NC, prec);
-    if (prec == MXM_PREC_FP64) {
-      Dt.upload(o.Dt.data(), o.Dt.size());
-      Lt.upload(o.Lt.data(), o.Lt.size());
-    } else {
-      Aop.upload(o.Aop.data(), o.Aop.size());
-    }
-    w.upload(re.w.data(), re.w.size());
-    Dn.upload(re.Dn.data(), re.Dn.size());
-    fmap.upload(re.fmap.data(), re.fmap.size());
-    dprm.upload(prm, MXM_PRM_MAX);
     MXM_HIP(hipDeviceSynchronize());
+  }
+
+  // moves the state to tiles of size w (its values unchanged)
+  void set_tile(bool w) {
+    if (w == wide) return;
+    if (w && !has_wide()) throw ArgError("this shape and form have no wide tile");
+    if (limiter) materialize(0);
+    std::vector<double> h((size_t)dof());
+    download_state(h.data());
+    const bool w0 = wide;
+    try {
+      wide = w;
+      layout(*hmesh, *hre);
+      upload_state(h.data());
+    } catch (...) {
+      // back to the former tiling with the same state, then report
+      wide = w0;
+      layout(*hmesh, *hre);
+      upload_state(h.data());
+      launch = find(cfg, false);
+      throw;
+    }
+    launch = find(cfg, false);
   }
 
   // Shu-Osher plan of src/ssprk3.mojo:41-78 (FP64 coefficients).
@@ -204,7 +248,7 @@ struct Solver {
 
   // the stage kernel of the launch configuration, in its limited variant
   // when the limiter is on
-  StageLaunch kernel() const { return limiter ? find_stage(dim, phys, variant, P, prec, cfg, true) : launch; }
+  StageLaunch kernel() const { return limiter ? find(cfg, true) : launch; }
 
   // writes the limited values of stage buffer b (0 = q) and clears its
   // pending factors
@@ -219,22 +263,73 @@ struct Solver {
     }
   }
 
-  // times every launch configuration on SSPRK3 stage 2 (q1 and q -> q2,
-  // the work buffer, after one stage 1 q -> q1: neither q nor q1 changes
-  // while it is timed; it reads q_a as two of the three stages do) and
-  // keeps the fastest
-  int autotune(double dt, int reps, double *us) {
-    MXM_HIP(hipStreamSynchronize(stream));
-    if (prec == MXM_PREC_FP64) {
-      stage_with(0, dt, q.p, q1.p, q2.p);
-      const StageArgs<double> a = stage_args(1, dt, q.p, q1.p, q2.p);
-      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us, limiter);
-    } else {
-      stage_with(0, dt, f.p, f1.p, f2.p);
-      const StageArgs<float> a = stage_args(1, dt, f.p, f1.p, f2.p);
-      cfg = autotune_stage(dim, phys, variant, P, prec, &a, stream, reps, us, limiter);
+  // times every launch configuration of the current tiling on SSPRK3
+  // steps of the current state (reps untimed steps; then per configuration
+  // one untimed step and reps steps between two events on the solver's
+  // stream, stream launches), keeps the fastest
+  // and returns its time per stage; the state is saved to the host first
+  // and restored after.  Full steps, rather than one stage repeated on
+  // fixed inputs, see the cache state of the real cycle: on a state past
+  // the Infinity Cache the repeated stage ranks the configurations
+  // differently.
+  double tune(double dt, int reps, double *us) {
+    if (limiter) materialize(0);
+    std::vector<double> h((size_t)dof());
+    download_state(h.data());
+    hipEvent_t ev[2];
+    for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
+    struct Events {
+      hipEvent_t *e;
+      ~Events() {
+        for (int k = 0; k < 2; ++k) (void)hipEventDestroy(e[k]);
+      }
+    } own{ev};
+    int best = stage_cfg_default(prec);
+    double tbest = 0.0;
+    // reps untimed steps first: freshly allocated buffers and a device
+    // coming out of idle run the first steps slower, which would favour
+    // the configurations timed last
+    for (int r = 0; r < reps; ++r)
+      for (int st = 0; st < 3; ++st) stage(st, dt);
+    for (int c = 0; c < stage_cfg_count(prec); ++c) {
+      cfg = c;
+      launch = find(c, false);
+      if (!launch) throw ArgError("no stage kernel for a launch configuration");
+      for (int st = 0; st < 3; ++st) stage(st, dt);
+      MXM_HIP(hipEventRecord(ev[0], stream));
+      for (int r = 0; r < reps; ++r)
+        for (int st = 0; st < 3; ++st) stage(st, dt);
+      MXM_HIP(hipEventRecord(ev[1], stream));
+      MXM_HIP(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      MXM_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      const double t = (double)ms * 1e3 / (3.0 * reps);
+      if (us) us[c] = t;
+      if (c == 0 || t < tbest) best = c, tbest = t;
     }
-    launch = find_stage(dim, phys, variant, P, prec, cfg);
+    cfg = best;
+    launch = find(cfg, false);
+    upload_state(h.data());
+    return tbest;
+  }
+
+  // the fastest launch configuration, and for a shape with a wide tile
+  // the faster tiling: the state moves to the other tiling, its
+  // configurations are timed, and it moves back if none is faster
+  int autotune(double dt, int reps, double *us) {
+    const double t0 = tune(dt, reps, us);
+    if (!has_wide() || !hmesh) return cfg;
+    const int c0 = cfg;
+    double u1[kStageCfgMax] = {0};
+    set_tile(!wide);
+    const double t1 = tune(dt, reps, u1);
+    if (t1 < t0) {
+      if (us)
+        for (int k = 0; k < stage_cfg_count(prec); ++k) us[k] = u1[k];
+      return cfg;
+    }
+    cfg = c0;
+    set_tile(!wide);
     return cfg;
   }
 
@@ -780,12 +875,23 @@ int mxm_dg_launch_config(void *h, int cfg, int *current) {
   return guarded([&] {
     Solver *s = as_solver(h);
     if (cfg >= 0) {
-      StageLaunch l = find_stage(s->dim, s->phys, s->variant, s->P, s->prec, cfg);
+      StageLaunch l = s->find(cfg, false);
       if (!l) throw ArgError("launch configuration out of range");
       s->cfg = cfg;
       s->launch = l;
     }
     if (current) *current = s->cfg;
+    return MXM_OK;
+  });
+}
+
+int mxm_dg_tile_config(void *h, int wide, int *elems, int *threads) {
+  return guarded([&] {
+    Solver *s = as_solver(h);
+    if (wide > 1) throw ArgError("tile size is 0 (256 threads), 1 (wide) or -1 (query)");
+    if (wide >= 0) s->set_tile(wide == 1);
+    if (elems) *elems = s->til.epb;
+    if (threads) *threads = stage_block(s->dim, s->NP, s->NFP, s->NC, s->wide);
     return MXM_OK;
   });
 }

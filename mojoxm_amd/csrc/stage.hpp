@@ -9,8 +9,9 @@
 // (for 2D) the face-flux + vol_lift_combine_rk kernel pair of
 // src/local_mesh_2d_gpu_<phys>.mojo fused into one pass.
 //
-// One 256-thread block (4 wave64) per tile of at most EPB elements, which
-// are contiguous in device memory:
+// One block (4 wave64, or 5 for the wide 2D P = 3 tile of the FP32-MFMA
+// form, below) per tile of at most EPB elements, which are contiguous in
+// device memory:
 //   1. each thread starts the loads of its first face item (job record
 //      with the face geometry, partner node, and the partner's trace when
 //      the partner lies outside the tile); the block stages the q_in tile
@@ -96,7 +97,7 @@ struct StageArgs {
 constexpr int kPrmMax = 32;  // MXM_PRM_MAX
 
 // Launch configurations of the stage kernel: each is an occupancy target
-// (waves per SIMD, i.e. 256-thread blocks per CU) that bounds the register
+// (waves per SIMD, e.g. 256-thread blocks per CU) that bounds the register
 // allocation; all configurations compute the same bits.  The autotuner
 // times them on the problem and keeps the fastest.  FP64 kernels need
 // 100-150 VGPRs: 5 waves (<= 102 VGPRs) or 4 (<= 128, which takes the
@@ -118,26 +119,54 @@ struct StageCfg<float> {
 };
 constexpr int kStageCfgMax = 4;
 
-// Elements per 256-thread block: at most 256/NP for NP nodes per element,
-// rounded down to whole lattice cells (2 triangles, 6 tets) when a cell
-// fits, then reduced -- by cells, then to runs of elements -- until the
-// block's LDS at FP64 (the operand array -- the q_in tile and the face
-// fluxes, later the ND*NP flux planes, rows of EPB*NC + 1 values -- and
-// the geometry) fits 64 KB, which only the two-fluid state (NC = 17)
-// needs.  Tiles are blocks of cells, so the LDS is sized for what a
-// tile can hold; the host builds the tiles with the same value.
+// Elements per tile (one thread per element node): at most 256/NP for NP
+// nodes per element, rounded down to whole lattice cells (2 triangles, 6
+// tets) when a cell fits, then reduced -- by cells, then to runs of
+// elements -- until the block's LDS at FP64 (the operand array -- the q_in
+// tile and the face fluxes, later the ND*NP flux planes, rows of
+// EPB*NC + 1 values -- and the geometry) fits 64 KB, which only the
+// two-fluid state (NC = 17) needs.  Tiles are blocks of cells, so the LDS
+// is sized for what a tile can hold; the host builds the tiles with the
+// same value.
+//
+// The wide tile (wide = true; the FP32-MFMA form only, where the solver's
+// autotuner times both tile sizes and keeps the faster): when 256 threads
+// leave more than 5 % of the lanes without a node and whole cells fill a
+// block of 5 to 8 waves exactly -- of the shapes built, 2D P = 3 (NP = 10:
+// 24 triangles on 240 of 256 lanes, or 32 on all 320 lanes of 5 waves).
+// It pays when all tiles are resident at once: a mesh of 128 x 128
+// squares makes 1,024 wide tiles, 4 per CU and 5 waves per SIMD, against
+// 1,376 narrow ones, 5 or 6 per CU and 6 waves on the busiest SIMDs.
+// When tiles queue for the CUs the narrow tile is faster: a 5-wave block
+// puts two of its waves on one SIMD.  The FP64 contraction layout
+// (stage_groups) is sized for 4 waves and has no wide tile.
 constexpr int stage_lds_bytes(int ND, int NP, int NFP, int NC, int epb) {
   return (epb * NP * NC + (ND + 1) * NFP * (epb * NC + 1) > ND * NP * (epb * NC + 1)
               ? epb * NP * NC + (ND + 1) * NFP * (epb * NC + 1)
               : ND * NP * (epb * NC + 1)) * 8 +
          epb * (ND * ND + 1) * 8 + (ND + 1) * NFP * 4;
 }
-constexpr int stage_epb(int ND, int NP, int NFP, int NC) {
+constexpr int stage_epb(int ND, int NP, int NFP, int NC, bool wide = false) {
   const int cell = ND == 3 ? 6 : 2;
   int e = 256 / NP;
   if (e >= cell) e -= e % cell;
+  if (wide && e >= cell && e * NP * 20 < 256 * 19)
+    for (int w = 5; w <= 8; ++w) {
+      const int t = 64 * w;
+      if (t % NP == 0 && (t / NP) % cell == 0 && t / NP <= 126 &&
+          stage_lds_bytes(ND, NP, NFP, NC, t / NP) <= 65536)
+        return t / NP;
+    }
   while (e > 1 && stage_lds_bytes(ND, NP, NFP, NC, e) > 65536) e = e > cell ? e - cell : e - 1;
   return e;
+}
+// Threads per block: 256, or the whole waves of a wide tile
+constexpr int stage_block(int ND, int NP, int NFP, int NC, bool wide = false) {
+  return stage_epb(ND, NP, NFP, NC, wide) * NP > 256 ? (stage_epb(ND, NP, NFP, NC, wide) * NP + 63) / 64 * 64 : 256;
+}
+// whether the shape has a wide tile distinct from the 256-thread one
+constexpr bool stage_has_wide(int ND, int NP, int NFP, int NC) {
+  return stage_epb(ND, NP, NFP, NC, true) != stage_epb(ND, NP, NFP, NC, false);
 }
 
 // FP64 contraction layout: the tile's (element, component) columns span
@@ -170,14 +199,20 @@ constexpr int stage_operand_len(bool mfma, int TILE, int NCOLS, int KV, int KVP,
               : cmax(TILE + NFF * NCOLS, KV * NCOLS);
 }
 
-template <int ND, int P, int NC = 1>
+template <int ND, int P, int NC = 1, bool WIDE = false>
 struct Shape {
   static constexpr int NP = ND == 3 ? (P + 1) * (P + 2) * (P + 3) / 6 : (P + 1) * (P + 2) / 2;
   static constexpr int NFP = ND == 3 ? (P + 1) * (P + 2) / 2 : P + 1;
   static constexpr int NF = ND + 1;
-  static constexpr int BLOCK = 256;
-  static constexpr int EPB = stage_epb(ND, NP, NFP, NC);
+  static constexpr int BLOCK = stage_block(ND, NP, NFP, NC, WIDE);
+  static constexpr int EPB = stage_epb(ND, NP, NFP, NC, WIDE);
+  static constexpr bool kHasWide = stage_has_wide(ND, NP, NFP, NC);
 };
+// the stage kernel's shape for the state type R (the wide tile: FP32 only)
+template <class Phys, int ND, int P, class R, bool WIDE>
+using StageShape = Shape<ND, P, Phys::NC, WIDE && std::is_same<R, float>::value>;
+template <class Phys, int ND, int P, class R, bool WIDE>
+constexpr int kStageBlock = StageShape<Phys, ND, P, R, WIDE>::BLOCK;
 
 // Bijective map from the hardware block index (dispatched round-robin over
 // the 8 XCDs) to a tile index such that XCD x owns a contiguous tile range.
@@ -274,10 +309,14 @@ __device__ __forceinline__ float sr_f32(double x, unsigned salt) {
 
 // W: occupancy target of the launch configuration, in waves per SIMD
 // (amdgpu_waves_per_eu); 1 leaves the register budget to the compiler.
-template <class Phys, int ND, int P, class R, int W, bool LIM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void dg_stage_kernel(StageArgs<R> A) {
-  using S = Shape<ND, P, Phys::NC>;
+// WIDE: the wide tile (stage_epb).
+template <class Phys, int ND, int P, class R, int W, bool LIM, bool WIDE>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, kStageBlock<Phys, ND, P, R, WIDE>),
+                          amdgpu_waves_per_eu(W))) void
+dg_stage_kernel(StageArgs<R> A) {
+  using S = StageShape<Phys, ND, P, R, WIDE>;
   constexpr bool kMFMA = std::is_same<R, float>::value;
+  static_assert(kMFMA || S::BLOCK == 256, "the FP64 contraction layout is sized for 4 waves");
   constexpr int NP = S::NP, NFP = S::NFP, NF = S::NF, EPB = S::EPB, NC = Phys::NC;
   constexpr int TILE = EPB * NP * NC;  // values of one element tile
   constexpr int NCOL = EPB * NC;       // (element, component) columns of a tile
@@ -388,7 +427,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
     for (int c = 0; c < NC; ++c) qa_in[c] = g[c];
   }
   const int nE = A.tile_count[tile];
-  // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= 256
+  // thread tid owns output node (el, i) = (tid / NP, tid % NP): EPB*NP <= BLOCK
   const bool node = tid < nE * NP;
   if (LIM && !qa_is_in && tid < EPB * NP) {
     const size_t e = (size_t)e0 + tid / NP;
@@ -541,7 +580,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   // columns [D (KVP) | Lift (KPAD - KVP)]; V_MFMA_F32_16X16X4F32 operands,
   // lane l = 16 g + r: A[r][k0 + g], B[k0 + g][col0 + r]; result rows
   // 4 g + v (v < 4), column col0 + r.
-  constexpr int NTW = (NT + 3) / 4;  // column tiles per wave
+  constexpr int NTW = (NT + S::BLOCK / 64 - 1) / (S::BLOCK / 64);  // column tiles per wave
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int l = tid & 63, lr = l & 15, lg = l >> 4;
   mxm_float4 macc[kMFMA ? MT : 1][kMFMA ? NTW : 1];
@@ -746,14 +785,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void d
   }
 }
 
-template <class Phys, int ND, int P, class R, int W, bool LIM>
+// A shape without a distinct wide tile launches its one kernel for WIDE.
+template <class Phys, int ND, int P, class R, int W, bool LIM, bool WIDE>
 hipError_t launch_stage(const void *args, hipStream_t s) {
-  using S = Shape<ND, P, Phys::NC>;
-  const StageArgs<R> &a = *static_cast<const StageArgs<R> *>(args);
-  const int nb = a.ntiles;
-  if (nb <= 0) return hipSuccess;
-  hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R, W, LIM>), dim3(nb), dim3(S::BLOCK), 0, s, a);
-  return hipGetLastError();
+  constexpr bool kWide = WIDE && std::is_same<R, float>::value && Shape<ND, P, Phys::NC>::kHasWide;
+  if constexpr (WIDE && !kWide) {
+    return launch_stage<Phys, ND, P, R, W, LIM, false>(args, s);
+  } else {
+    using S = StageShape<Phys, ND, P, R, WIDE>;
+    const StageArgs<R> &a = *static_cast<const StageArgs<R> *>(args);
+    const int nb = a.ntiles;
+    if (nb <= 0) return hipSuccess;
+    hipLaunchKernelGGL((dg_stage_kernel<Phys, ND, P, R, W, LIM, WIDE>), dim3(nb), dim3(S::BLOCK), 0, s, a);
+    return hipGetLastError();
+  }
 }
 
 // args points to StageArgs<double> (FP64 form) or StageArgs<float>

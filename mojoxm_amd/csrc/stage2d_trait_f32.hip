@@ -2,7 +2,7 @@
 #include "stage_families.hpp"
 
 namespace mxm {
-StageLaunch stage_2d_trait_fp32(int phys, int variant, int P, int cfg, bool lim) {
-  return pick_order<Family2DTrait, float>(phys, variant, P, cfg, lim);
+StageLaunch stage_2d_trait_fp32(int phys, int variant, int P, int cfg, bool lim, bool wide) {
+  return pick_order<Family2DTrait, float>(phys, variant, P, cfg, lim, wide);
 }
 }  // namespace mxm

@@ -169,9 +169,11 @@ class Solver:
 
     def autotune(self, dt: float, reps: int = 20):
         """Time every launch configuration of the stage kernel on this mesh
-        (stage 2 of SSPRK3 into a work buffer, after one stage 1, which
-        leaves the state unchanged) and keep the fastest.  Returns (chosen
-        index, microseconds per configuration)."""
+        (reps SSPRK3 steps of the current state per configuration, after
+        untimed warm-up steps; the state is saved and restored) and keep the fastest; for a shape with a
+        wide tile (`tile`), both tile sizes are timed and the faster tiling
+        kept.  Returns (chosen index, microseconds per stage of each
+        configuration of the kept tiling)."""
         us = (C.c_double * 4)()
         cfg = C.c_int()
         L.check(L.lib().mxm_dg_autotune(self._h, dt, reps, C.byref(cfg), us, 4), "mxm_dg_autotune")
@@ -186,6 +188,21 @@ class Solver:
     @launch_config.setter
     def launch_config(self, cfg: int) -> None:
         L.check(L.lib().mxm_dg_launch_config(self._h, int(cfg), None), "mxm_dg_launch_config")
+
+    @property
+    def tile(self) -> tuple:
+        """(wide, elements per tile, threads per block) of the stage kernel's
+        tiling; wide tiles exist for the FP32-MFMA form on triangles at
+        P = 3."""
+        e, t = C.c_int(), C.c_int()
+        L.check(L.lib().mxm_dg_tile_config(self._h, -1, C.byref(e), C.byref(t)), "mxm_dg_tile_config")
+        return (t.value > 256, e.value, t.value)
+
+    @tile.setter
+    def tile(self, wide: bool) -> None:
+        """Lay the state out in wide tiles (True) or 256-thread tiles; the
+        state keeps its values."""
+        L.check(L.lib().mxm_dg_tile_config(self._h, 1 if wide else 0, None, None), "mxm_dg_tile_config")
 
     def time_stage_kernel(self, dt: float, nsteps: int) -> float:
         """Average stage-kernel duration in microseconds (HIP events on the
