@@ -837,6 +837,9 @@ struct Batch {
     }
     MXM_HIP(hipStreamEndCapture(cstream, &out.graph));
     MXM_HIP(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
+    // the executable goes to the device now, not at its first launch, on
+    // the stream that launches it
+    MXM_HIP(hipGraphUpload(out.exec, g.stream));
   }
   void capture(BGroup &g) {
     capture_steps(g, BGroup::kGraphSteps, g.sg[0]);

@@ -116,6 +116,8 @@ struct Solver {
     }
     MXM_HIP(hipStreamEndCapture(stream, &out.graph));
     MXM_HIP(hipGraphInstantiate(&out.exec, out.graph, nullptr, nullptr, 0));
+    // the executable goes to the device now, not at its first launch
+    MXM_HIP(hipGraphUpload(out.exec, stream));
   }
 
   void build(HostMesh m, const RefElem &re) {
