@@ -268,24 +268,37 @@ struct Solver {
   // times every launch configuration of the current tiling on SSPRK3
   // steps of the current state (reps untimed steps; then per configuration
   // one untimed step and reps steps between two events on the solver's
-  // stream, stream launches), keeps the fastest
-  // and returns its time per stage; the state is saved to the host first
-  // and restored after.  Full steps, rather than one stage repeated on
-  // fixed inputs, see the cache state of the real cycle: on a state past
-  // the Infinity Cache the repeated stage ranks the configurations
-  // differently.
+  // stream, stream launches), keeps the fastest and returns its time per
+  // stage.  The state is saved to the host first and restored after, also
+  // when a call fails.  Full steps, rather than one stage repeated on fixed
+  // inputs, see the cache state of the real cycle: on a state past the
+  // Infinity Cache the repeated stage ranks the configurations differently.
   double tune(double dt, int reps, double *us) {
     if (limiter) materialize(0);
     std::vector<double> h((size_t)dof());
     download_state(h.data());
-    hipEvent_t ev[2];
-    for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
+    const int c0 = cfg;
+    try {
+      const double t = time_configs(dt, reps, us);
+      upload_state(h.data());
+      return t;
+    } catch (...) {
+      cfg = c0;
+      launch = find(cfg, false);
+      upload_state(h.data());
+      throw;
+    }
+  }
+  double time_configs(double dt, int reps, double *us) {
+    hipEvent_t ev[2] = {nullptr, nullptr};
     struct Events {
       hipEvent_t *e;
       ~Events() {
-        for (int k = 0; k < 2; ++k) (void)hipEventDestroy(e[k]);
+        for (int k = 0; k < 2; ++k)
+          if (e[k]) (void)hipEventDestroy(e[k]);
       }
     } own{ev};
+    for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
     int best = stage_cfg_default(prec);
     double tbest = 0.0;
     // reps untimed steps first: freshly allocated buffers and a device
@@ -311,7 +324,6 @@ struct Solver {
     }
     cfg = best;
     launch = find(cfg, false);
-    upload_state(h.data());
     return tbest;
   }
 
