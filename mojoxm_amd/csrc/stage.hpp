@@ -137,9 +137,11 @@ constexpr int kStageCfgMax = 4;
 // It pays when all tiles are resident at once: a mesh of 128 x 128
 // squares makes 1,024 wide tiles, 4 per CU and 5 waves per SIMD, against
 // 1,376 narrow ones, 5 or 6 per CU and 6 waves on the busiest SIMDs.
-// When tiles queue for the CUs the narrow tile is faster: a 5-wave block
-// puts two of its waves on one SIMD.  The FP64 contraction layout
-// (stage_groups) is sized for 4 waves and has no wide tile.
+// When tiles queue for the CUs the narrow tile is faster (an 8-wave tile
+// of 50 triangles measured slower still): a CU then holds fewer, larger
+// blocks, whose barriers idle more waves at once and whose phases overlap
+// less.  The FP64 contraction layout (stage_groups) is sized for 4 waves
+// and has no wide tile.
 constexpr int stage_lds_bytes(int ND, int NP, int NFP, int NC, int epb) {
   return (epb * NP * NC + (ND + 1) * NFP * (epb * NC + 1) > ND * NP * (epb * NC + 1)
               ? epb * NP * NC + (ND + 1) * NFP * (epb * NC + 1)
