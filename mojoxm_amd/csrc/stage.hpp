@@ -338,7 +338,17 @@ dg_stage_kernel(StageArgs<R> A) {
   constexpr int KPAD = kMFMA ? mfma_kpad(ND, NP, NFP) : KV + NFF;
   constexpr int MT = mfma_mtiles(NP);
   constexpr int NT = (NCOL + 15) / 16;
-  constexpr int NB = stage_operand_len(kMFMA, TILE, NCOLS, KV, KVP, NFF, KPAD, NT * 16);
+  // FP32 form of a 2D state of 7 or more components (GLM-MHD), without the
+  // limiter: q_a is loaded with the tile into LDS past the operand array
+  // and read by the node threads in 2b, instead of waiting in NC registers
+  // through the face phase, the register peak.  GLM-MHD then fits 6 waves
+  // per SIMD without spills and its 384^2 stage takes 2 % less; the Euler
+  // kernels lost 1 % (their registers were not the limit) and the 3D
+  // kernels would give LDS that bounds their blocks per CU.
+  constexpr bool kQaLds = kMFMA && !LIM && ND == 2 && NC >= 7;
+  constexpr int NB0 = stage_operand_len(kMFMA, TILE, NCOLS, KV, KVP, NFF, KPAD, NT * 16);
+  constexpr int QA0 = NB0;  // past every operand read and the zeroed padding
+  constexpr int NB = NB0 + (kQaLds ? TILE : 0);
   // FP32-MFMA: the operator [D | Lift] staged in LDS when it is at most
   // 16 KB (2D, 3D P <= 3), else read from global memory (L2) in the loop
   constexpr int NA = kMFMA ? MT * 16 * KPAD : 0;
@@ -388,6 +398,16 @@ dg_stage_kernel(StageArgs<R> A) {
     for (int k = 0; k < QPT; ++k)
       if (k * S::BLOCK + tid < TILE) tq[k] = src[k * S::BLOCK + tid];
   }
+  const bool qa_is_in = A.q_a == A.q_in;
+  R tqa[kQaLds ? QPT : 1];
+  if constexpr (kQaLds) {
+    if (!qa_is_in) {
+      const R *src = A.q_a + (size_t)e0 * NP * NC;
+#pragma unroll
+      for (int k = 0; k < QPT; ++k)
+        if (k * S::BLOCK + tid < TILE) tqa[k] = src[k * S::BLOCK + tid];
+    }
+  }
   double tg[GPT];
 #pragma unroll
   for (int k = 0; k < GPT; ++k) {
@@ -419,11 +439,11 @@ dg_stage_kernel(StageArgs<R> A) {
   }
   FaceItem<R, NC> it;
   if (tid < nitems) load_item_records<NFP>(A, tile, tid, e0, it);
-  // q_a of the own node, in the state precision until 2b; when q_a is the
-  // stage input (SSPRK3 stage 1) it is read from the LDS copy of the tile
-  const bool qa_is_in = A.q_a == A.q_in;
+  // q_a of the own node (from LDS in 2b when kQaLds), in the state
+  // precision until 2b; when q_a is the stage input (SSPRK3 stage 1) it is
+  // read from the LDS copy of the tile
   R qa_in[NC];
-  if (tid < EPB * NP && !qa_is_in) {
+  if (!kQaLds && tid < EPB * NP && !qa_is_in) {
     const R *g = A.q_a + (size_t)e0 * NP * NC + (size_t)tid * NC;
 #pragma unroll
     for (int c = 0; c < NC; ++c) qa_in[c] = g[c];
@@ -447,6 +467,13 @@ dg_stage_kernel(StageArgs<R> A) {
     if (t < NG) sJ[t] = tg[k];
   }
   if (tid < NF * NFP) sMap[tid] = fm;
+  if constexpr (kQaLds) {
+    if (!qa_is_in) {
+#pragma unroll
+      for (int k = 0; k < QPT; ++k)
+        if (k * S::BLOCK + tid < TILE) sB[QA0 + k * S::BLOCK + tid] = tqa[k];
+    }
+  }
   if constexpr (kStageA) {
 #pragma unroll
     for (int k = 0; k < APT; ++k) {
@@ -455,7 +482,7 @@ dg_stage_kernel(StageArgs<R> A) {
     }
   }
   if constexpr (kMFMA)
-    for (int t = TILE + NFF * NCOLS + tid; t < NB; t += S::BLOCK) sB[t] = R(0);
+    for (int t = TILE + NFF * NCOLS + tid; t < NB0; t += S::BLOCK) sB[t] = R(0);
   if constexpr (LIM) {
 #pragma unroll
     for (int k = 0; k < LPT; ++k)
@@ -564,6 +591,12 @@ dg_stage_kernel(StageArgs<R> A) {
     for (int c = 0; c < NC; ++c) src[c] = R(0);
     if constexpr (Phys::kSource) {
       if (phys.source_active()) phys.source(q, src);
+    }
+    if constexpr (kQaLds) {
+      if (!qa_is_in) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) qa_in[c] = sB[QA0 + tid * NC + c];
+      }
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
