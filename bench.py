@@ -454,7 +454,11 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=20)
+    # 1000 C2 steps are about 30 ms of work: a device coming out of idle
+    # (after the autotuning's host work and the state upload) clocks up over
+    # 20-30 ms, and 400 steps timed right after 20 warmup steps read 3-4 %
+    # slower than the same steps on a warm device
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=480, help="oracle steps timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the full run to T=2 (for profiling)")

@@ -222,8 +222,10 @@ int mxm_dg_memory_report(void *solver, mxm_memory_report *out);
  * back-to-back launches (limiter off), from two HIP events on the solver's
  * own stream around nsteps SSPRK3 steps launched as mxm_dg_step_ssprk3
  * launches them (replayed HIP graphs), so the average includes the
- * launch-to-launch gaps; advances the state by nsteps + 1 steps.  No
- * reference counterpart: it is the measurement hook that the roofline
+ * launch-to-launch gaps.  Before them, untimed: about 30 ms of steps (a
+ * device coming out of idle clocks up over that much work) and the same
+ * nsteps steps once, so the state advances by those steps and 2*nsteps.
+ * No reference counterpart: it is the measurement hook that the roofline
  * figures are computed from. */
 int mxm_dg_time_stage_kernel(void *solver, double dt, int nsteps, double *avg_us);
 /* Domain diagnostics of the current state, computed on the device:

@@ -180,6 +180,23 @@ struct Solver {
     MXM_HIP(hipDeviceSynchronize());
   }
 
+  // Untimed steps (stream launches) until about `ms` milliseconds of them
+  // have run: after an idle gap (a state upload, host work) the device
+  // clocks up over 20-30 ms of work, and steps timed sooner read up to 20 %
+  // slower (the 32^3 Taylor-Green stage: 224 us right after an upload, 196
+  // after 17 ms of steps, 191 after 60 ms).
+  static constexpr double kWarmUpMs = 30.0;
+  void warm_up(double dt, double ms) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 1, total = 0; total < 65536; total += k, k = k < 256 ? 2 * k : 256) {
+      for (int j = 0; j < k; ++j)
+        for (int st = 0; st < 3; ++st) stage(st, dt);
+      MXM_HIP(hipStreamSynchronize(stream));
+      if (std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() >= ms) break;
+    }
+    if (limiter) materialize(0);
+  }
+
   // moves the state to tiles of size w (its values unchanged)
   void set_tile(bool w) {
     if (w == wide) return;
@@ -266,8 +283,8 @@ struct Solver {
   }
 
   // times every launch configuration of the current tiling on SSPRK3
-  // steps of the current state (reps untimed steps; then per configuration
-  // one untimed step and reps steps between two events on the solver's
+  // steps of the current state (a warm-up; then per configuration one
+  // untimed step and reps steps between two events on the solver's
   // stream, stream launches), keeps the fastest and returns its time per
   // stage.  The state is saved to the host first and restored after, also
   // when a call fails.  Full steps, rather than one stage repeated on fixed
@@ -301,11 +318,9 @@ struct Solver {
     for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
     int best = stage_cfg_default(prec);
     double tbest = 0.0;
-    // reps untimed steps first: freshly allocated buffers and a device
-    // coming out of idle run the first steps slower, which would favour
-    // the configurations timed last
-    for (int r = 0; r < reps; ++r)
-      for (int st = 0; st < 3; ++st) stage(st, dt);
+    // a device coming out of idle runs the first steps slower, which would
+    // favour the configurations timed last
+    warm_up(dt, kWarmUpMs);
     for (int c = 0; c < stage_cfg_count(prec); ++c) {
       cfg = c;
       launch = find(c, false);
@@ -854,10 +869,11 @@ int mxm_dg_time_stage_kernel(void *h, double dt, int nsteps, double *avg_us) {
     s->limiter = false;
     // two events on the solver's stream around nsteps SSPRK3 steps, launched
     // as the stepping loop launches them (replayed graphs): the average
-    // includes the launch-to-launch gaps.  The graphs are built, and one
-    // step run, before the timed region.
+    // includes the launch-to-launch gaps.  Before the timed region the
+    // device is warmed up and the same graphs run once, untimed.
     s->prepare(dt, nsteps);
-    s->step(dt, 1);
+    s->warm_up(dt, Solver::kWarmUpMs);
+    s->step(dt, nsteps);
     hipEvent_t ev[2];
     for (auto &e : ev) MXM_HIP(hipEventCreate(&e));
     MXM_HIP(hipEventRecord(ev[0], s->stream));

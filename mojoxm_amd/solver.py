@@ -207,7 +207,8 @@ class Solver:
     def time_stage_kernel(self, dt: float, nsteps: int) -> float:
         """Average stage-kernel duration in microseconds (HIP events on the
         solver's stream around nsteps steps, limiter off for them and
-        restored after); advances the state by nsteps + 1 steps."""
+        restored after), after about 30 ms of untimed warm-up steps and the
+        same nsteps steps once; advances the state by all of them."""
         v = C.c_double()
         L.check(L.lib().mxm_dg_time_stage_kernel(self._h, dt, nsteps, C.byref(v)), "mxm_dg_time_stage_kernel")
         return v.value
