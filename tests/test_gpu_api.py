@@ -154,3 +154,24 @@ def test_time_stage_kernel_restores_limiter():
         a.step_ssprk3(dt, 2)
         b.step_ssprk3(dt, 2)
         assert np.array_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("limiter", [False, True])
+def test_prepare_steps_leaves_the_state_and_the_steps_unchanged(limiter):
+    # prepare_steps builds graphs without stepping: the state is unchanged,
+    # and the steps that follow give the bits of a solver that never
+    # prepared
+    rng = np.random.default_rng(23)
+    dt = 1e-3
+    a, q0 = make(CASES[0], M.PREC_FP64, rng)
+    b, _ = make(CASES[0], M.PREC_FP64, rng)
+    with a, b:
+        for s in (a, b):
+            s.set_state(q0)
+            if limiter:
+                s.enable_cell_limiter(True, 0.1)
+        a.prepare_steps(dt, 37)
+        assert np.array_equal(a.get_state(), q0)
+        a.step_ssprk3(dt, 37)
+        b.step_ssprk3(dt, 37)
+        assert np.array_equal(a.get_state(), b.get_state())
