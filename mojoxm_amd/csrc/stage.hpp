@@ -338,14 +338,17 @@ dg_stage_kernel(StageArgs<R> A) {
   constexpr int KPAD = kMFMA ? mfma_kpad(ND, NP, NFP) : KV + NFF;
   constexpr int MT = mfma_mtiles(NP);
   constexpr int NT = (NCOL + 15) / 16;
-  // FP32 form of a 2D state of 7 or more components (GLM-MHD), without the
-  // limiter: q_a is loaded with the tile into LDS past the operand array
-  // and read by the node threads in 2b, instead of waiting in NC registers
-  // through the face phase, the register peak.  GLM-MHD then fits 6 waves
-  // per SIMD without spills and its 384^2 stage takes 2 % less; the Euler
-  // kernels lost 1 % (their registers were not the limit) and the 3D
-  // kernels would give LDS that bounds their blocks per CU.
-  constexpr bool kQaLds = kMFMA && !LIM && ND == 2 && NC >= 7;
+  // 2D kernels without the limiter, FP32 form with 7 or more components
+  // (GLM-MHD) and FP64 form with at most 5: q_a is loaded with the tile
+  // into LDS past the operand array and read by the node threads in 2b,
+  // instead of waiting in NC registers through the face phase, the
+  // register peak.  FP32 GLM-MHD then fits 6 waves per SIMD without spills
+  // (its 384^2 stage 2 % faster) and the FP64 Euler kernels 5 (96 VGPRs,
+  // 31.8 KB LDS: the 512^2 vortex stage 4 % faster, C2 0.5 %).  The FP32
+  // Euler kernels lost 1 % (their registers were not the limit), FP64
+  // GLM-MHD and Maxwell would become LDS-bound, and the 3D kernels' LDS
+  // would bound their blocks per CU.
+  constexpr bool kQaLds = !LIM && ND == 2 && (kMFMA ? NC >= 7 : NC <= 5);
   constexpr int NB0 = stage_operand_len(kMFMA, TILE, NCOLS, KV, KVP, NFF, KPAD, NT * 16);
   constexpr int QA0 = NB0;  // past every operand read and the zeroed padding
   constexpr int NB = NB0 + (kQaLds ? TILE : 0);
